@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the likelihood hot path on one MI355X.
+
+Workload (the headline configuration): 2,000,000 accept-rejected data events x
+88 waves (8 with reflectivity -1, 80 with +1), rank 2, normalisation integral
+from a 5,000,000-event MC pool, all generated on the GPU with the synthetic
+amplitude recipe.  A step is one FdF (value + full gradient, the NLopt call)
+of rpwa::pwaLikelihood with complex64 amplitudes (mixed precision: FP64
+arithmetic, FP64 compensated event sum); the same number of FdF calls is timed
+on the all-FP64 drop-in (complex128 amplitudes) for comparison.  Every result
+is checked against the CPU oracle at the full size.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("lnL+grad evals/s, 2M ev×88 waves (mixed vs FP64, |ΔlnL|<1e-3); fits/hr")
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
+SEED = 20261202 + 1
+STREAM_TRUE_T, STREAM_MC, STREAM_DATA, STREAM_START = 2, 3, 4, 5
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def wave_list(nW):
+    return [("neg%02d" % i, -1, 0.0) for i in range(nW[0])] + [("pos%02d" % i, +1, 0.0) for i in range(nW[1])]
+
+
+def start_values(rng, nPars, nEvents, W):
+    """Random start at the scale of the fit (|T|^2 ~ N/W per wave)."""
+    return rng.normal(size=nPars) * np.sqrt(nEvents / W)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--events", type=int, default=2_000_000)
+    ap.add_argument("--mc-events", type=int, default=5_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=400_000, help="events of the single-thread CPU baseline")
+    ap.add_argument("--cpu-evals", type=int, default=3)
+    ap.add_argument("--oracle-threads", type=int, default=16)
+    ap.add_argument("--skip-oracle", action="store_true", help="skip the full-size oracle check (profiling runs)")
+    ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path shards over events with one "
+              "fixed-order exchange of ~350 doubles per evaluation; no multi-GPU figure is produced); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from rootpwa_amd import capi
+    from rootpwa_amd import likelihood as PL
+
+    if capi.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = capi.device_info(0)
+    log("device", dev)
+
+    nW = (8, 80)
+    W = sum(nW)
+    rank = 2
+    N, M = args.events, args.mc_events
+    waves = wave_list(nW)
+    names = [w[0] for w in waves]
+
+    # ---- workload generation on the GPU --------------------------------------
+    t0 = time.perf_counter()
+    scales = capi.gen_wave_scales(SEED, W)
+    truth = capi.gen_prod_amps(SEED, STREAM_TRUE_T, rank, nW)
+    d_mc = capi.DeviceBuffer(M * W * 16)
+    capi.gen_amps_device(d_mc.address, SEED, STREAM_MC, 0, M, scales)
+    log("MC pool generated: %d events (%.1f s)" % (M, time.perf_counter() - t0))
+    # normalisation integral on FP64 MFMA (first call warms up)
+    sums = capi.integral_device(d_mc.address, M, W)
+    capi.synchronize()
+    t1 = time.perf_counter()
+    n_int = 3
+    for _ in range(n_int):
+        sums = capi.integral_device(d_mc.address, M, W)
+    t_int = (time.perf_counter() - t1) / n_int
+    int_flops = 8.0 * M * W * (W + 1) / 2
+    log("integral: %.3f ms per call, %.1f TFLOP/s algorithmic" % (1e3 * t_int, int_flops / t_int / 1e12))
+    d_mc.free()
+    d_data = capi.DeviceBuffer(N * W * 16)
+    t1 = time.perf_counter()
+    pool, smax = capi.gen_accept_reject_device(d_data.address, SEED, STREAM_DATA, N, nW, rank, truth, scales)
+    log("data: %d events accepted from a pool of %d (%.1f s)" % (N, pool, time.perf_counter() - t1))
+
+    intmat = PL.ampIntegralMatrix(names)
+    intmat.setSums(sums, M)
+
+    def make(precision):
+        L = PL.pwaLikelihood(precision)
+        L.init(waves, rank)
+        L.addNormIntegral(intmat)
+        L.addAccIntegral(intmat)
+        L.addAmplitudesDevice(d_data.address, N)
+        L.finishInit()
+        return L
+
+    L_mix = make(capi.PREC_MIXED)
+    L_64 = make(capi.PREC_FP64)
+    nPars = L_mix.nmbPars()
+    rng = np.random.default_rng(SEED + STREAM_START)
+    par = start_values(rng, nPars, N, W)
+
+    # ---- timed FdF evaluations ------------------------------------------------
+    def timed(L, label):
+        ctx = L.gpuContext()
+        for _ in range(args.warmup):
+            L.FdF(par)
+        capi.synchronize()
+        ctx.reset_kernel_time()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            f, g = L.FdF(par)
+        capi.synchronize()
+        el = time.perf_counter() - t
+        ev, red, n = ctx.kernel_times()
+        info = ctx.info()
+        rate = args.steps / el
+        k_avg = ev / max(n, 1)
+        log("%s: %.1f evals/s (%.3f ms per FdF; event kernel %.3f ms, reduce %.3f ms; tile %d lanes x %d slots, "
+            "%d workgroups)" % (label, rate, 1e3 * el / args.steps, 1e3 * k_avg, 1e3 * red / max(n, 1),
+                                info["lanes"], info["slots"], info["workgroups"]))
+        return dict(rate=rate, ms=1e3 * el / args.steps, kernel_s=k_avg, reduce_s=red / max(n, 1), f=f, g=g,
+                    info=info)
+
+    r_mix = timed(L_mix, "mixed (complex64)")
+    r_64 = timed(L_64, "fp64 (complex128)")
+
+    # ---- checks against the oracle at full size ---------------------------------
+    check = {}
+    if not args.skip_oracle:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        data = d_data.download((N, W), np.complex128)
+        Lo = O.OracleLikelihood(nW, rank)
+        Lo.add_integrals(sums, M)
+        Lo.add_amplitudes(data)
+        Lo.finish_init()
+        f_ref, g_ref = Lo.fdf(par, threads=args.oracle_threads)
+        del data
+        gscale = float(np.max(np.abs(g_ref)))
+        check = dict(
+            lnL_oracle=f_ref,
+            dlnL_mixed=abs(r_mix["f"] - f_ref), dlnL_fp64=abs(r_64["f"] - f_ref),
+            rel_lnL_fp64=abs(r_64["f"] - f_ref) / abs(f_ref),
+            grad_rel_mixed=float(np.max(np.abs(r_mix["g"] - g_ref))) / gscale,
+            grad_rel_fp64=float(np.max(np.abs(r_64["g"] - g_ref))) / gscale)
+        log("oracle check (%.1f s, %d threads): %s" % (time.perf_counter() - t, args.oracle_threads, check))
+        del Lo
+
+    # ---- CPU baseline: the oracle, single thread, on a bounded sample ------------
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        S = min(args.cpu_sample, N)
+        sub = d_data.download((S, W), np.complex128)
+        Lc = O.OracleLikelihood(nW, rank)
+        Lc.add_integrals(sums, M)
+        Lc.add_amplitudes(sub)
+        Lc.finish_init()
+        del sub
+        Lc.fdf(par)
+        t = time.perf_counter()
+        for _ in range(args.cpu_evals):
+            Lc.fdf(par)
+        el = (time.perf_counter() - t) / args.cpu_evals
+        cpu = dict(value=(1.0 / el) * S / N, unit="evals/s", cores=1, kind="port",
+                   sample="%d FdF calls of the single-thread C oracle (restatement of pwaLikelihood::FdF, "
+                          "Kahan sums, per-event allocation) on the first %d of the %d events, %.2f s each; "
+                          "evals/s scaled by %d/%d to the full event count" % (args.cpu_evals, S, N, el, S, N))
+        log("cpu baseline:", cpu)
+
+    bytes_mix = float(N) * W * 8
+    bytes_64 = float(N) * W * 16
+    ach_mix = bytes_mix / r_mix["kernel_s"] / 1e9
+    ach_64 = bytes_64 / r_64["kernel_s"] / 1e9
+    out = {
+        "metric": METRIC,
+        "value": r_mix["rate"],
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r_mix["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "amplitude_storage": "complex64",
+        "data": "synthetic (GPU-generated amplitudes, accept-reject pseudo data)",
+        "config": {
+            "workload": "2M accept-rejected events x 88 waves (8 refl -1 + 80 refl +1), rank 2, "
+                        "5M-event MC normalisation; one pwaLikelihood::FdF per step, complex64 amplitudes",
+            "events": N, "waves": W, "waves_refl": list(nW), "rank": rank, "mc_events": M, "parameters": nPars,
+            "data_pool": pool, "seed": SEED, "device": dev["arch"], "cus": dev["cus"],
+        },
+        "roofline": {"bound": "hbm", "achieved": ach_mix, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_mix / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "event_kernel<mixed> (fused lnL + gradient)",
+                     "kernel_ms": 1e3 * r_mix["kernel_s"], "bytes_per_launch": bytes_mix},
+        "fp64": {"evals_per_s": r_64["rate"], "ms_per_eval": r_64["ms"], "kernel_ms": 1e3 * r_64["kernel_s"],
+                 "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS},
+        "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
+        "integral": {"ms": 1e3 * t_int, "tflops_algorithmic": int_flops / t_int / 1e12,
+                     "frac_fp64_mfma": int_flops / t_int / 1e12 / FP64_MFMA_PEAK_TFS, "mc_events": M},
+        "check": check,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu"] = r_mix["rate"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

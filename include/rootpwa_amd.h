@@ -1,0 +1,192 @@
+/*
+ * rootpwa_amd.h -- C ABI of the MI355X (gfx950) backend for ROOTPWA's unbinned
+ * extended log-likelihood.
+ *
+ * This is the lower boundary of rpwa::pwaLikelihood: it replaces
+ * rpwa::cuda::likelihoodInterface<cuda::complex<T>> (cuda/likelihoodInterface.cuh:72-166),
+ * which the reference calls from partialWaveFit/pwaLikelihood.cc:352-357 (DoEval),
+ * :533-541 (Gradient) and :1357-1377 (finishInit).  Differences from the
+ * reference interface, all deliberate:
+ *   - a context handle instead of a static singleton, so several likelihoods
+ *     (and Clone()s sharing one context through rpwa_lik_retain) can coexist;
+ *   - every entry point returns a status code and never throws or exits;
+ *     rpwa_last_error() returns the message of the last failure on this thread;
+ *   - 64-bit sizes (the reference's `unsigned int` byte counts overflow above
+ *     4 GiB, likelihoodInterface.cu:236);
+ *   - a fused value+gradient entry point (rpwa_lik_fdf) for pwaLikelihood::FdF,
+ *     which has no GPU branch in the reference.
+ * All calls are host-synchronous: results are in host memory on return.
+ * Complex numbers are interleaved (re, im) doubles, the layout of
+ * std::complex<double>.
+ */
+#ifndef ROOTPWA_AMD_H
+#define ROOTPWA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RPWA_OK            0
+#define RPWA_ERR_INVALID   1  /* bad argument (null pointer, size, rank ...) */
+#define RPWA_ERR_STATE     2  /* call order violated (e.g. eval before amplitudes) */
+#define RPWA_ERR_HIP       3  /* HIP runtime error (message in rpwa_last_error) */
+#define RPWA_ERR_NOMEM     4  /* device or pinned host allocation failed */
+#define RPWA_ERR_UNSUPPORTED 5 /* shape outside the built kernel variants */
+
+/* amplitude storage / arithmetic of a likelihood context */
+#define RPWA_PREC_FP64     0  /* complex128 amplitudes, FP64 arithmetic: the plain drop-in */
+#define RPWA_PREC_MIXED    1  /* complex64 amplitudes (rounded after FP64 normalisation),
+                                 FP64 arithmetic, FP64 compensated event sum */
+
+/* host layouts accepted by rpwa_lik_load_decay_amps */
+#define RPWA_LAYOUT_REFL_WAVE_EVT 0 /* [2][nmbWavesMax][nmbEvents], zero padded: the array
+                                        pwaLikelihood::finishInit builds for the GPU
+                                        (pwaLikelihood.cc:1357-1374) */
+#define RPWA_LAYOUT_EVT_WAVE      1 /* [nmbEvents][nmbWavesRefl[0] + nmbWavesRefl[1]]:
+                                        refl -1 waves first, then refl +1 waves */
+
+typedef struct rpwa_lik_ctx rpwa_lik_ctx;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* rpwa_last_error(void);
+
+/* Number of visible GPUs (0 when none); never fails. */
+int rpwa_device_count(void);
+
+/* Create a context on `device` (replaces likelihoodInterface::initCudaDevice,
+ * likelihoodInterface.cu:141).  precision: RPWA_PREC_*. */
+int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision);
+/* Add a reference to a context (pwaLikelihood::Clone shares the device state). */
+int rpwa_lik_retain(rpwa_lik_ctx* ctx);
+/* Drop a reference; the last one frees the device memory
+ * (replaces likelihoodInterface::closeCudaDevice, likelihoodInterface.cu:180). */
+int rpwa_lik_release(rpwa_lik_ctx* ctx);
+
+/* Upload the (already normalised) decay amplitudes once
+ * (replaces likelihoodInterface::init / loadDecayAmps, likelihoodInterface.cu:119,195).
+ * nmbDecayAmps = number of complex values in `decayAmps` for the given layout. */
+int rpwa_lik_load_decay_amps(rpwa_lik_ctx* ctx, const double* decayAmps, uint64_t nmbDecayAmps,
+                             uint64_t nmbEvents, const uint32_t nmbWavesRefl[2], int layout);
+
+/* Same, from amplitudes already in device memory on the context's device, in
+ * RPWA_LAYOUT_EVT_WAVE order.  waveScale (host, [W0+W1], may be NULL) multiplies
+ * each wave's column in FP64 before the amplitudes are stored (the
+ * 1/sqrt(Re I_ii) normalisation of pwaLikelihood.cc:1253-1260). */
+int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* d_amps, uint64_t nmbEvents,
+                                    const uint32_t nmbWavesRefl[2], const double* waveScale);
+
+/* Event part of -lnL: sum_e -ln( sum_r sum_refl |sum_i T_r,refl,i A_refl,i(e)|^2 + flat^2 )
+ * (replaces likelihoodInterface::logLikelihood, likelihoodInterface.cuh:112-115).
+ * prodAmps: [rank][2][nmbWavesMax] complex, nmbProdAmps = rank*2*nmbWavesMax. */
+int rpwa_lik_log_likelihood(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                            double prodAmpFlat, uint32_t rank, double* logLikelihood);
+
+/* Event part of the gradient: derivatives[r][refl][i] = sum_e -(2/sigma_e) S_r,refl(e)
+ * conj(A_refl,i(e)), derivativeFlat = sum_e -(2/sigma_e) flat
+ * (replaces likelihoodInterface::logLikelihoodDeriv, likelihoodInterface.cuh:117-122). */
+int rpwa_lik_log_likelihood_deriv(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                                  double prodAmpFlat, uint32_t rank, double* derivatives,
+                                  double* derivativeFlat);
+
+/* Both event parts from one pass over the amplitudes.  No counterpart in the
+ * reference (FdF has no GPU branch, pwaLikelihood.cc:172-315).  The value is
+ * bitwise identical to rpwa_lik_log_likelihood's at the same prodAmps. */
+int rpwa_lik_fdf(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                 double prodAmpFlat, uint32_t rank, double* logLikelihood,
+                 double* derivatives, double* derivativeFlat);
+
+/* Event part of the Hessian (pwaLikelihood.cc:697-753): for a, b over the
+ * [rank][2][nmbWavesMax] production amplitudes, hessian[a][b][3] holds the
+ * real/real, real/imag and imag/imag event sums; flatTerms[a] (complex) the
+ * mixed flat-wave terms and hessianFlat the flat/flat term. */
+int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                     double prodAmpFlat, uint32_t rank, double* hessian, double* flatTerms,
+                     double* hessianFlat);
+
+/* Accumulated time spent in this context's kernels [s]
+ * (likelihoodInterface::kernelTime / resetKernelTime, likelihoodInterface.cuh:124-125). */
+int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds);
+int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx);
+
+/* Split of the accumulated kernel time: event kernel and fixed-order
+ * reduction [s], and the number of evaluations they cover. */
+int rpwa_lik_kernel_times(rpwa_lik_ctx* ctx, double* eventKernel, double* reduceKernel,
+                          uint64_t* nmbEvaluations);
+
+/* Shape and launch configuration of a context:
+ * info[0] events, [1] W(refl -1), [2] W(refl +1), [3] precision, [4] lanes per event,
+ * [5] slots per lane, [6] workgroups, [7] threads per workgroup, [8] bytes of amplitudes. */
+int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
+
+/* Override the event-tile shape (lanes per event, slots per lane) chosen at
+ * load time; used by the tile autotuner.  lanes in {8,16,32,64}. */
+int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
+
+/* Stream on which the context launches its kernels (for event timing). */
+int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream);
+
+/* ---------------------------------------------------------------------------
+ * Device memory helpers, so that a caller needs no second GPU runtime to stage
+ * amplitudes (tests, the benchmark, C++ drivers).  No counterpart in the
+ * reference (its CUDA backend only copies host arrays, likelihoodInterface.cu:236-239).
+ * --------------------------------------------------------------------------- */
+#define RPWA_COPY_H2D 0
+#define RPWA_COPY_D2H 1
+#define RPWA_COPY_D2D 2
+int rpwa_dev_alloc(int device, uint64_t bytes, void** ptr);
+int rpwa_dev_free(void* ptr);
+int rpwa_dev_copy(void* dst, const void* src, uint64_t bytes, int kind);
+int rpwa_dev_synchronize(int device);
+/* name (NUL-terminated, truncated to nameLen), compute units, global memory
+ * bytes, peak engine clock [kHz] */
+int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* totalMem, int* clockKhz);
+
+/* ---------------------------------------------------------------------------
+ * Normalisation integrals: ampIntegralMatrix::integrate
+ * (decayAmplitude/ampIntegralMatrix.cc:300-488) for amplitudes with one
+ * incoherent sub-amplitude, as an FP64-MFMA Hermitian rank-k update.
+ * outSums[W][W] (complex): sum_e weight_e A_i(e) conj(A_j(e)), weight_e = 1/w_e
+ * for importance weights w (or 1), renormalised by the mean weight when weights
+ * are given (ampIntegralMatrix.cc:477-483).  The caller divides by the number
+ * of events, as ampIntegralMatrix::element does.
+ * --------------------------------------------------------------------------- */
+/* amplitudes in host memory, [M][W] complex; streamed through the device */
+int rpwa_integral(int device, const double* amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                  const double* weights, double* outSums);
+/* amplitudes (and weights) already in device memory */
+int rpwa_integral_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                         const double* d_weights, double* outSums);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic amplitudes (the generator behind the benchmark workloads).
+ * A_i(e) = (N(0,1) + i N(0,1)) * s_i * exp(0.5 N(0,1)_e), drawn from a
+ * counter-based hash of (seed, stream, event, slot), so any event of a pool
+ * can be regenerated independently.
+ * --------------------------------------------------------------------------- */
+/* s_i ~ U[0.1, 1] for i < nmbWaves (host computation) */
+int rpwa_gen_wave_scales(uint64_t seed, uint32_t nmbWaves, double* scales);
+/* complex-normal production amplitudes [rank][2][Wmax] with the rank
+ * restrictions of pwaLikelihood (T=0 for i<r, real for i=r) */
+int rpwa_gen_prod_amps(uint64_t seed, uint32_t stream, uint32_t rank, const uint32_t nmbWavesRefl[2],
+                       double* prodAmps);
+/* events [eventOffset, eventOffset+nmbEvents) of pool `stream` into device
+ * memory d_out [nmbEvents][nmbWaves] */
+int rpwa_gen_amps_device(int device, uint64_t seed, uint32_t stream, uint64_t eventOffset,
+                         uint64_t nmbEvents, uint32_t nmbWaves, const double* scales, double* d_out);
+/* accept-reject: the first nmbWanted events of pool `stream` accepted with
+ * probability sigma_e / max(sigma) under prodAmps ([rank][2][Wmax]), the
+ * maximum taken over the pool; the pool grows (x2) until enough are accepted.
+ * Writes the accepted amplitudes to d_out [nmbWanted][W0+W1] (device) and,
+ * if not NULL, the pool size and the maximum intensity used. */
+int rpwa_gen_accept_reject_device(int device, uint64_t seed, uint32_t stream, uint64_t nmbWanted,
+                                  const uint32_t nmbWavesRefl[2], uint32_t rank,
+                                  const double* prodAmps, const double* scales, double* d_out,
+                                  uint64_t* poolSize, double* maxIntensity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+/*
+ * rootpwa_amd_pwalik.h -- C ABI over the C++ likelihood class rpwa::pwaLikelihood
+ * (include/rpwa/pwaLikelihood.h) and rpwa::ampIntegralMatrix.
+ *
+ * It exposes what ROOTPWA's Python binding of the likelihood exposes
+ * (pyInterface/bindings/partialWaveFit/pwaLikelihood_py.cc:243-312: init,
+ * addNormIntegral, addAccIntegral, addAmplitude, finishInit, Gradient, FdF,
+ * DoEval, DoDerivative, Hessian, CovarianceMatrix, CorrectParamSigns, the
+ * accessors and the prior settings), so a foreign-language caller (ctypes,
+ * cgo, JNI) reaches the same object a C++ minimiser driver uses.  Arrays are
+ * plain pointers with explicit sizes; complex values are interleaved doubles.
+ * Every function returns a status code (RPWA_OK = 0, see rootpwa_amd.h);
+ * rpwa_last_error() describes the last failure.
+ */
+#ifndef ROOTPWA_AMD_PWALIK_H
+#define ROOTPWA_AMD_PWALIK_H
+
+#include <stdint.h>
+
+#include "rootpwa_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rpwa_intmat rpwa_intmat;   /* rpwa::ampIntegralMatrix */
+typedef struct rpwa_pwalik rpwa_pwalik;   /* rpwa::pwaLikelihood<std::complex<T>> */
+
+/* ---- ampIntegralMatrix (decayAmplitude/ampIntegralMatrix.h:67-172) ---- */
+int rpwa_intmat_create(rpwa_intmat** m);
+int rpwa_intmat_destroy(rpwa_intmat* m);
+/* names: nmbWaves NUL-terminated strings */
+int rpwa_intmat_set_wave_names(rpwa_intmat* m, uint32_t nmbWaves, const char* const* names);
+int rpwa_intmat_nmb_waves(const rpwa_intmat* m, uint32_t* nmbWaves);
+/* name of wave i, NUL-terminated, truncated to nameLen */
+int rpwa_intmat_wave_name(const rpwa_intmat* m, uint32_t i, char* name, int nameLen);
+/* sums [W][W] complex and the number of events they cover (W = rpwa_intmat_nmb_waves) */
+int rpwa_intmat_set_sums(rpwa_intmat* m, const double* sums, uint64_t nmbEvents);
+int rpwa_intmat_get_sums(const rpwa_intmat* m, double* sums, uint64_t* nmbEvents);
+int rpwa_intmat_set_nmb_events(rpwa_intmat* m, uint64_t nmbEvents);
+/* element(i, j) = sum_ij / nmbEvents (ampIntegralMatrix.cc:238-252) */
+int rpwa_intmat_element(const rpwa_intmat* m, uint32_t i, uint32_t j, double* re, double* im);
+/* ampIntegralMatrix::integrate on the GPU: amps [M][W] complex in host
+ * (onDevice = 0) or device (onDevice = 1) memory, weights (importance weights,
+ * same memory space) or NULL */
+int rpwa_intmat_integrate(rpwa_intmat* m, const double* amps, uint64_t nmbEvents, const double* weights,
+                          int device, int onDevice);
+int rpwa_intmat_add(rpwa_intmat* m, const rpwa_intmat* other);   /* operator+= */
+int rpwa_intmat_renormalize(rpwa_intmat* m, uint64_t nmbEvents);
+int rpwa_intmat_write_ascii(const rpwa_intmat* m, const char* fileName);
+int rpwa_intmat_read_ascii(rpwa_intmat* m, const char* fileName);
+
+/* ---- pwaLikelihood (partialWaveFit/pwaLikelihood.h:66-323) ---- */
+/* precision: RPWA_PREC_FP64 -> pwaLikelihood<complex<double>>,
+ *            RPWA_PREC_MIXED -> pwaLikelihood<complex<float>> (complex64 amplitudes) */
+int rpwa_pwalik_create(rpwa_pwalik** L, int precision);
+int rpwa_pwalik_clone(const rpwa_pwalik* L, rpwa_pwalik** out);   /* Clone(): shares the GPU context */
+int rpwa_pwalik_destroy(rpwa_pwalik* L);
+int rpwa_pwalik_set_device(rpwa_pwalik* L, int device);
+/* init(waveDescThres, rank, massBinCenter): reflectivities are +1 / -1 */
+int rpwa_pwalik_init(rpwa_pwalik* L, uint32_t nmbWaves, const char* const* names, const int* reflectivities,
+                     const double* thresholds, uint32_t rank, double massBinCenter);
+int rpwa_pwalik_use_normalized_amps(rpwa_pwalik* L, int flag);
+int rpwa_pwalik_set_prior(rpwa_pwalik* L, int halfCauchy, double cauchyWidth);
+int rpwa_pwalik_add_norm_integral(rpwa_pwalik* L, const rpwa_intmat* m);
+int rpwa_pwalik_add_acc_integral(rpwa_pwalik* L, rpwa_intmat* m, uint32_t accEventsOverride);
+int rpwa_pwalik_add_amplitude(rpwa_pwalik* L, const char* waveName, const double* amps, uint64_t nmbEvents);
+/* all waves at once from device memory, [N][W] complex128: the refl -1 waves,
+ * then the refl +1 waves, each group in init() wave-list order */
+int rpwa_pwalik_add_amplitudes_device(rpwa_pwalik* L, const double* d_amps, uint64_t nmbEvents);
+int rpwa_pwalik_finish_init(rpwa_pwalik* L);
+
+int rpwa_pwalik_nmb_pars(const rpwa_pwalik* L, uint32_t* nmbPars, uint32_t* nmbParsFixed);
+int rpwa_pwalik_nmb_events(const rpwa_pwalik* L, uint64_t* nmbEvents);
+int rpwa_pwalik_nmb_waves(const rpwa_pwalik* L, int reflectivity, uint32_t* nmbWaves);
+/* parameter i: name (parName(), truncated to nameLen), rank, fixed, realPart */
+int rpwa_pwalik_parameter(const rpwa_pwalik* L, uint32_t i, char* name, int nameLen, uint32_t* rank,
+                          int* fixed, int* realPart);
+
+int rpwa_pwalik_doeval(const rpwa_pwalik* L, const double* par, double* value);
+int rpwa_pwalik_gradient(const rpwa_pwalik* L, const double* par, double* gradient);
+int rpwa_pwalik_fdf(const rpwa_pwalik* L, const double* par, double* value, double* gradient);
+int rpwa_pwalik_doderivative(const rpwa_pwalik* L, const double* par, uint32_t k, double* value);
+/* nmbPars x nmbPars row-major */
+int rpwa_pwalik_hessian(const rpwa_pwalik* L, const double* par, double* hessian);
+int rpwa_pwalik_covariance(const rpwa_pwalik* L, const double* par, double* covariance);
+int rpwa_pwalik_correct_param_signs(const rpwa_pwalik* L, const double* par, double* out);
+/* [rank][2][nmbWavesReflMax] complex and the flat amplitude */
+int rpwa_pwalik_copy_from_par(const rpwa_pwalik* L, const double* par, double* prodAmps, double* flat);
+int rpwa_pwalik_copy_to_par(const rpwa_pwalik* L, const double* prodAmps, double flat, double* par);
+/* integralMatrices(withFlat): (W[+1])^2 complex each, phase-space integrals W[+1] */
+int rpwa_pwalik_integral_matrices(const rpwa_pwalik* L, int withFlat, double* normMatrix, double* accMatrix,
+                                  double* phaseSpaceIntegral);
+/* the event-sum context of a finished likelihood (borrowed; shared with its
+ * clones): kernel timing, tile selection (rpwa_lik_* of rootpwa_amd.h) */
+int rpwa_pwalik_gpu_context(const rpwa_pwalik* L, rpwa_lik_ctx** ctx);
+/* number of calls and accumulated total time [s] of FdF, Gradient, DoEval,
+ * DoDerivative, Hessian (functionCallEnum order) */
+int rpwa_pwalik_call_info(const rpwa_pwalik* L, uint32_t calls[5], double totalTime[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

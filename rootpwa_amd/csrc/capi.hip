@@ -1,0 +1,644 @@
+// capi.hip -- likelihood contexts behind the C ABI of rootpwa_amd.h.
+//
+// A context owns the device copy of the decay amplitudes of one likelihood,
+// the staging buffers of the production amplitudes and results, and the fixed
+// launch geometry that makes every evaluation's reductions deterministic.
+// It replaces the static singleton state of cuda::likelihoodInterface
+// (cuda/likelihoodInterface.cu:94-107): the amplitudes are uploaded once
+// (loadDecayAmps, likelihoodInterface.cu:195-241), nothing is allocated per
+// evaluation (the reference allocates and frees device memory in every
+// logLikelihood / logLikelihoodDeriv call, likelihoodInterface.cu:378-415),
+// and the host waits once per evaluation instead of after every kernel.
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace rpwa_amd {
+
+static thread_local std::string g_lastError;
+
+void set_error(const std::string& msg) { g_lastError = msg; }
+void clear_error() { g_lastError.clear(); }
+
+namespace {
+
+int ilog2(int v)
+{
+	int r = 0;
+	while ((1 << r) < v)
+		++r;
+	return r;
+}
+
+uint32_t sector_slots(uint32_t nw, int lanes, uint32_t perSlot)
+{
+	const uint32_t per = (uint32_t)lanes * perSlot;
+	return (nw + per - 1) / per;
+}
+
+// Tile for a given lane count; false if the row does not fit the register tiles.
+bool make_tile(const AmpLayout& lay, int lanes, TileShape& ts)
+{
+	const uint32_t s0 = sector_slots(lay.nW[0], lanes, lay.perSlot);
+	const uint32_t s1 = sector_slots(lay.nW[1], lanes, lay.perSlot);
+	const uint32_t slots = s0 + s1;
+	if (slots == 0 || slots > 8)
+		return false;
+	ts.lanes = lanes;
+	ts.slots = (int)slots;
+	ts.slot1 = (int)s0;
+	ts.kmax = slots <= 2 ? 2 : (slots <= 4 ? 4 : 8);
+	return true;
+}
+
+// Default tile: minimise the modelled VALU cycles per event (complex MACs of
+// the slots, the per-event log/division and the butterfly), rank 2 assumed.
+bool choose_tile(const AmpLayout& lay, TileShape& best)
+{
+	double bestCost = 1e300;
+	bool found = false;
+	for (int lanes : {8, 16, 32, 64}) {
+		TileShape ts;
+		if (!make_tile(lay, lanes, ts))
+			continue;
+		const double perWave = ts.slots * (double)lay.perSlot * 2 * 8 + 40 + 8.0 * 2 * ilog2(lanes) +
+		                       4.0 * (ts.kmax - ts.slots);
+		const double cost = perWave * lanes / 64.0;
+		if (cost < bestCost) {
+			bestCost = cost;
+			best = ts;
+			found = true;
+		}
+	}
+	return found;
+}
+
+int check_ctx(rpwa_lik_ctx* ctx, const char* fn)
+{
+	if (!ctx) {
+		set_error(std::string(fn) + ": null context");
+		return RPWA_ERR_INVALID;
+	}
+	return RPWA_OK;
+}
+
+void free_buffers(rpwa_lik_ctx* c)
+{
+	hipFree(c->d_amps);
+	hipFree(c->d_prod);
+	hipFree(c->d_partials);
+	hipFree(c->d_out);
+	hipHostFree(c->h_prod);
+	hipHostFree(c->h_out);
+	c->d_amps = nullptr;
+	c->d_prod = nullptr;
+	c->d_partials = nullptr;
+	c->d_out = nullptr;
+	c->h_prod = nullptr;
+	c->h_out = nullptr;
+	c->loaded = false;
+}
+
+// (Re)compute the persistent grid for the context's tile.
+int configure_grid(rpwa_lik_ctx* c)
+{
+	int wgPerCu = 1;
+	int st = event_kernel_occupancy(c->lay, c->tile, true, &wgPerCu);
+	if (st != RPWA_OK) {
+		set_error("hipOccupancyMaxActiveBlocksPerMultiprocessor failed");
+		return st;
+	}
+	c->nWg = c->nCu * std::min(wgPerCu, 8);
+	const size_t need = (size_t)c->nWg * partial_elems(kMaxRank, c->lay.wmax);
+	if (need > c->partialCap) {
+		hipFree(c->d_partials);
+		c->d_partials = nullptr;
+		RPWA_HIP_CHECK(hipMalloc(&c->d_partials, need * sizeof(double)));
+		c->partialCap = need;
+	}
+	return RPWA_OK;
+}
+
+int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
+{
+	AmpLayout& L = c->lay;
+	L.nEvents = nEvents;
+	L.nW[0] = nW[0];
+	L.nW[1] = nW[1];
+	L.wmax = std::max(nW[0], nW[1]);
+	L.precision = c->precision;
+	L.perSlot = c->precision == RPWA_PREC_MIXED ? 2 : 1;
+	const uint32_t pad0 = (nW[0] + L.perSlot - 1) / L.perSlot * L.perSlot;
+	const uint32_t pad1 = (nW[1] + L.perSlot - 1) / L.perSlot * L.perSlot;
+	L.secOff[0] = 0;
+	L.secOff[1] = pad0;
+	L.rowLen = pad0 + pad1;
+	if (!choose_tile(L, c->tile)) {
+		set_error("rpwa_lik: " + std::to_string(nW[0]) + "+" + std::to_string(nW[1]) +
+		          " waves exceed the register tiles (at most " + std::to_string(8 * 64 * L.perSlot) +
+		          " waves in total)");
+		return RPWA_ERR_UNSUPPORTED;
+	}
+	return RPWA_OK;
+}
+
+int alloc_eval_buffers(rpwa_lik_ctx* c)
+{
+	const size_t nProd = (size_t)kMaxRank * 2 * c->lay.wmax;
+	const size_t nOut = partial_elems(kMaxRank, c->lay.wmax);
+	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
+	RPWA_HIP_CHECK(hipMalloc(&c->d_out, nOut * sizeof(double)));
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocDefault));
+	return configure_grid(c);
+}
+
+// load from FP64 amplitudes already on the device
+int load_from_device(rpwa_lik_ctx* c, const double* d_src, int srcLayout, uint64_t nEvents,
+                     const uint32_t nW[2], const double* waveScale)
+{
+	free_buffers(c);
+	c->partialCap = 0;
+	int st = setup_layout(c, nEvents, nW);
+	if (st != RPWA_OK)
+		return st;
+	RPWA_HIP_CHECK(hipMalloc(&c->d_amps, std::max<size_t>(c->lay.bytes(), 16)));
+	double* d_scale = nullptr;
+	if (waveScale) {
+		const uint32_t W = nW[0] + nW[1];
+		RPWA_HIP_CHECK(hipMalloc(&d_scale, W * sizeof(double)));
+		RPWA_HIP_CHECK(hipMemcpy(d_scale, waveScale, W * sizeof(double), hipMemcpyHostToDevice));
+	}
+	hipError_t e = launch_transform_amps(c->lay, d_src, srcLayout, d_scale, c->d_amps, c->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(c->stream);
+	hipFree(d_scale);
+	RPWA_HIP_CHECK(e);
+	st = alloc_eval_buffers(c);
+	if (st != RPWA_OK)
+		return st;
+	c->loaded = true;
+	return RPWA_OK;
+}
+
+// One evaluation: flags select what is copied back.
+int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
+             double* lnl, double* deriv, double* derivFlat, const char* fn)
+{
+	if (!c->loaded) {
+		set_error(std::string(fn) + ": decay amplitudes not loaded");
+		return RPWA_ERR_STATE;
+	}
+	if (!prodAmps || rank == 0 || rank > (uint32_t)kMaxRank) {
+		set_error(std::string(fn) + ": production amplitudes missing or rank outside 1.." +
+		          std::to_string(kMaxRank));
+		return rank > (uint32_t)kMaxRank ? RPWA_ERR_UNSUPPORTED : RPWA_ERR_INVALID;
+	}
+	const uint32_t wmax = c->lay.wmax;
+	if (nmbProdAmps != rank * 2 * wmax) {
+		set_error(std::string(fn) + ": nmbProdAmps must be rank*2*nmbWavesMax = " +
+		          std::to_string(rank * 2 * wmax));
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(c->device));
+	std::memcpy(c->h_prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2));
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, (size_t)nmbProdAmps * sizeof(double2),
+	                              hipMemcpyHostToDevice, c->stream));
+	const uint32_t nElem = partial_elems((int)rank, wmax);
+	EventArgs a;
+	a.amps = c->d_amps;
+	a.nEvents = c->lay.nEvents;
+	a.rowLen = c->lay.rowLen;
+	a.secOff0 = c->lay.secOff[0];
+	a.secOff1 = c->lay.secOff[1];
+	a.nW0 = c->lay.nW[0];
+	a.nW1 = c->lay.nW[1];
+	a.wmax = wmax;
+	a.rank = (int)rank;
+	a.slots = c->tile.slots;
+	a.slot1 = c->tile.slot1;
+	a.prod = reinterpret_cast<const double2*>(c->d_prod);
+	a.flat2 = flat * flat;
+	a.partials = c->d_partials;
+	a.nElem = nElem;
+	RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+	RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile, a, true, c->nWg, c->stream));
+	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
+	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg, nElem, flat, c->d_out, c->stream));
+	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+	const size_t nRes = deriv ? (nElem - 1) : 2;
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->h_out, c->d_out, nRes * sizeof(double), hipMemcpyDeviceToHost,
+	                              c->stream));
+	RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
+	float ms = 0.f, ms1 = 0.f;
+	if (hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
+	    hipEventElapsedTime(&ms1, c->evMid, c->ev1) == hipSuccess) {
+		c->eventTime += 1e-3 * ms;
+		c->reduceTime += 1e-3 * ms1;
+		c->kernelTime += 1e-3 * (ms + ms1);
+		++c->nEvals;
+	}
+	if (lnl)
+		*lnl = c->h_out[0];
+	if (derivFlat)
+		*derivFlat = c->h_out[1];
+	if (deriv)
+		std::memcpy(deriv, c->h_out + 2, (size_t)nmbProdAmps * sizeof(double2));
+	return RPWA_OK;
+}
+
+}  // namespace
+}  // namespace rpwa_amd
+
+using namespace rpwa_amd;
+
+#define RPWA_GUARD_BEGIN try {
+#define RPWA_GUARD_END                                  \
+	}                                                   \
+	catch (const std::bad_alloc&) {                     \
+		set_error("host allocation failed");            \
+		return RPWA_ERR_NOMEM;                          \
+	}                                                   \
+	catch (const std::exception& ex) {                  \
+		set_error(ex.what());                           \
+		return RPWA_ERR_INVALID;                        \
+	}
+
+extern "C" const char* rpwa_last_error(void) { return g_lastError.c_str(); }
+
+extern "C" int rpwa_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess)
+		return 0;
+	return n;
+}
+
+extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
+{
+	clear_error();
+	if (!ctx) {
+		set_error("rpwa_lik_create: null output pointer");
+		return RPWA_ERR_INVALID;
+	}
+	*ctx = nullptr;
+	if (precision != RPWA_PREC_FP64 && precision != RPWA_PREC_MIXED) {
+		set_error("rpwa_lik_create: unknown precision");
+		return RPWA_ERR_INVALID;
+	}
+	const int n = rpwa_device_count();
+	if (device < 0 || device >= n) {
+		set_error("rpwa_lik_create: no GPU with index " + std::to_string(device) + " (" +
+		          std::to_string(n) + " visible)");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	rpwa_lik_ctx* c = new rpwa_lik_ctx();
+	c->device = device;
+	c->precision = precision;
+	hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+	if (e == hipSuccess)
+		e = hipEventCreate(&c->ev0);
+	if (e == hipSuccess)
+		e = hipEventCreate(&c->ev1);
+	if (e == hipSuccess)
+		e = hipEventCreate(&c->evMid);
+	if (e == hipSuccess)
+		e = hipDeviceGetAttribute(&c->nCu, hipDeviceAttributeMultiprocessorCount, device);
+	if (e != hipSuccess) {
+		set_error(std::string("rpwa_lik_create: ") + hipGetErrorString(e));
+		if (c->stream)
+			hipStreamDestroy(c->stream);
+		delete c;
+		return RPWA_ERR_HIP;
+	}
+	*ctx = c;
+	return RPWA_OK;
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_retain(rpwa_lik_ctx* ctx)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_retain"))
+		return st;
+	ctx->refs.fetch_add(1);
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_release(rpwa_lik_ctx* ctx)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_release"))
+		return st;
+	if (ctx->refs.fetch_sub(1) == 1) {
+		hipSetDevice(ctx->device);
+		hipStreamSynchronize(ctx->stream);
+		free_buffers(ctx);
+		hipEventDestroy(ctx->ev0);
+		hipEventDestroy(ctx->ev1);
+		hipEventDestroy(ctx->evMid);
+		hipStreamDestroy(ctx->stream);
+		delete ctx;
+	}
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_load_decay_amps(rpwa_lik_ctx* ctx, const double* decayAmps, uint64_t nmbDecayAmps,
+                                        uint64_t nmbEvents, const uint32_t nmbWavesRefl[2], int layout)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_load_decay_amps"))
+		return st;
+	if (!decayAmps || !nmbWavesRefl || nmbEvents == 0 || nmbWavesRefl[0] + nmbWavesRefl[1] == 0) {
+		set_error("rpwa_lik_load_decay_amps: empty or null input");
+		return RPWA_ERR_INVALID;
+	}
+	const uint64_t W = (uint64_t)nmbWavesRefl[0] + nmbWavesRefl[1];
+	const uint64_t wmax = std::max(nmbWavesRefl[0], nmbWavesRefl[1]);
+	uint64_t expect = 0;
+	if (layout == RPWA_LAYOUT_EVT_WAVE)
+		expect = nmbEvents * W;
+	else if (layout == RPWA_LAYOUT_REFL_WAVE_EVT)
+		expect = 2 * wmax * nmbEvents;
+	else {
+		set_error("rpwa_lik_load_decay_amps: unknown layout");
+		return RPWA_ERR_INVALID;
+	}
+	if (nmbDecayAmps != expect) {
+		set_error("rpwa_lik_load_decay_amps: nmbDecayAmps = " + std::to_string(nmbDecayAmps) +
+		          " does not match the layout (" + std::to_string(expect) + ")");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
+	double* d_src = nullptr;
+	RPWA_HIP_CHECK(hipMalloc(&d_src, expect * sizeof(double2)));
+	hipError_t e = hipMemcpy(d_src, decayAmps, expect * sizeof(double2), hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		hipFree(d_src);
+		RPWA_HIP_CHECK(e);
+	}
+	const int st = load_from_device(ctx, d_src, layout, nmbEvents, nmbWavesRefl, nullptr);
+	hipFree(d_src);
+	return st;
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* d_amps, uint64_t nmbEvents,
+                                               const uint32_t nmbWavesRefl[2], const double* waveScale)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_load_decay_amps_device"))
+		return st;
+	if (!d_amps || !nmbWavesRefl || nmbEvents == 0 || nmbWavesRefl[0] + nmbWavesRefl[1] == 0) {
+		set_error("rpwa_lik_load_decay_amps_device: empty or null input");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
+	return load_from_device(ctx, d_amps, RPWA_LAYOUT_EVT_WAVE, nmbEvents, nmbWavesRefl, waveScale);
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_log_likelihood(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                                       double prodAmpFlat, uint32_t rank, double* logLikelihood)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_log_likelihood"))
+		return st;
+	if (!logLikelihood) {
+		set_error("rpwa_lik_log_likelihood: null output");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	return evaluate(ctx, prodAmps, nmbProdAmps, prodAmpFlat, rank, logLikelihood, nullptr, nullptr,
+	                "rpwa_lik_log_likelihood");
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_log_likelihood_deriv(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                                             double prodAmpFlat, uint32_t rank, double* derivatives,
+                                             double* derivativeFlat)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_log_likelihood_deriv"))
+		return st;
+	if (!derivatives || !derivativeFlat) {
+		set_error("rpwa_lik_log_likelihood_deriv: null output");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	return evaluate(ctx, prodAmps, nmbProdAmps, prodAmpFlat, rank, nullptr, derivatives, derivativeFlat,
+	                "rpwa_lik_log_likelihood_deriv");
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_fdf(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps, double prodAmpFlat,
+                            uint32_t rank, double* logLikelihood, double* derivatives, double* derivativeFlat)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_fdf"))
+		return st;
+	if (!logLikelihood || !derivatives || !derivativeFlat) {
+		set_error("rpwa_lik_fdf: null output");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	return evaluate(ctx, prodAmps, nmbProdAmps, prodAmpFlat, rank, logLikelihood, derivatives,
+	                derivativeFlat, "rpwa_lik_fdf");
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_kernel_time"))
+		return st;
+	if (!seconds) {
+		set_error("rpwa_lik_kernel_time: null output");
+		return RPWA_ERR_INVALID;
+	}
+	*seconds = ctx->kernelTime;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_reset_kernel_time"))
+		return st;
+	ctx->kernelTime = 0;
+	ctx->eventTime = 0;
+	ctx->reduceTime = 0;
+	ctx->nEvals = 0;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_kernel_times(rpwa_lik_ctx* ctx, double* eventKernel, double* reduceKernel,
+                                     uint64_t* nmbEvaluations)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_kernel_times"))
+		return st;
+	if (eventKernel)
+		*eventKernel = ctx->eventTime;
+	if (reduceKernel)
+		*reduceKernel = ctx->reduceTime;
+	if (nmbEvaluations)
+		*nmbEvaluations = ctx->nEvals;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9])
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_info"))
+		return st;
+	if (!info) {
+		set_error("rpwa_lik_info: null output");
+		return RPWA_ERR_INVALID;
+	}
+	info[0] = ctx->lay.nEvents;
+	info[1] = ctx->lay.nW[0];
+	info[2] = ctx->lay.nW[1];
+	info[3] = (uint64_t)ctx->precision;
+	info[4] = (uint64_t)ctx->tile.lanes;
+	info[5] = (uint64_t)ctx->tile.slots;
+	info[6] = (uint64_t)ctx->nWg;
+	info[7] = 256;
+	info[8] = ctx->lay.bytes();
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_set_tile"))
+		return st;
+	if (!ctx->loaded) {
+		set_error("rpwa_lik_set_tile: decay amplitudes not loaded");
+		return RPWA_ERR_STATE;
+	}
+	if (lanesPerEvent != 8 && lanesPerEvent != 16 && lanesPerEvent != 32 && lanesPerEvent != 64) {
+		set_error("rpwa_lik_set_tile: lanes per event must be 8, 16, 32 or 64");
+		return RPWA_ERR_INVALID;
+	}
+	TileShape ts;
+	if (!make_tile(ctx->lay, (int)lanesPerEvent, ts)) {
+		set_error("rpwa_lik_set_tile: the amplitude row does not fit " + std::to_string(lanesPerEvent) +
+		          " lanes x 8 slots");
+		return RPWA_ERR_UNSUPPORTED;
+	}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
+	const TileShape old = ctx->tile;
+	ctx->tile = ts;
+	const int st = configure_grid(ctx);
+	if (st != RPWA_OK)
+		ctx->tile = old;
+	return st;
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_stream"))
+		return st;
+	if (!hipStream) {
+		set_error("rpwa_lik_stream: null output");
+		return RPWA_ERR_INVALID;
+	}
+	*hipStream = ctx->stream;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                                double prodAmpFlat, uint32_t rank, double* hessian, double* flatTerms,
+                                double* hessianFlat)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_hessian"))
+		return st;
+	(void)prodAmps; (void)nmbProdAmps; (void)prodAmpFlat; (void)rank;
+	(void)hessian; (void)flatTerms; (void)hessianFlat;
+	set_error("rpwa_lik_hessian: not built yet");
+	return RPWA_ERR_UNSUPPORTED;
+}
+
+extern "C" int rpwa_dev_alloc(int device, uint64_t bytes, void** ptr)
+{
+	clear_error();
+	if (!ptr) {
+		set_error("rpwa_dev_alloc: null output");
+		return RPWA_ERR_INVALID;
+	}
+	*ptr = nullptr;
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+		*ptr = nullptr;
+		set_error("rpwa_dev_alloc: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+		return RPWA_ERR_NOMEM;
+	}
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_dev_free(void* ptr)
+{
+	clear_error();
+	RPWA_HIP_CHECK(hipFree(ptr));
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_dev_copy(void* dst, const void* src, uint64_t bytes, int kind)
+{
+	clear_error();
+	if ((!dst || !src) && bytes) {
+		set_error("rpwa_dev_copy: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	hipMemcpyKind k;
+	switch (kind) {
+	case RPWA_COPY_H2D: k = hipMemcpyHostToDevice; break;
+	case RPWA_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+	case RPWA_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+	default:
+		set_error("rpwa_dev_copy: unknown copy kind");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_dev_synchronize(int device)
+{
+	clear_error();
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	RPWA_HIP_CHECK(hipDeviceSynchronize());
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* totalMem, int* clockKhz)
+{
+	clear_error();
+	hipDeviceProp_t p;
+	RPWA_HIP_CHECK(hipGetDeviceProperties(&p, device));
+	if (name && nameLen > 0) {
+		std::strncpy(name, p.gcnArchName, (size_t)nameLen - 1);
+		name[nameLen - 1] = 0;
+	}
+	if (nCu)
+		*nCu = p.multiProcessorCount;
+	if (totalMem)
+		*totalMem = p.totalGlobalMem;
+	if (clockKhz)
+		*clockKhz = p.clockRate;
+	return RPWA_OK;
+}

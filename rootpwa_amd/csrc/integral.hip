@@ -1,0 +1,383 @@
+// integral.hip -- A.A^dagger normalisation integrals on FP64 MFMA (gfx950).
+//
+// ampIntegralMatrix::integrate (decayAmplitude/ampIntegralMatrix.cc:408-483)
+// accumulates I_ij = sum_e wt_e A_i(e) conj(A_j(e)).  With the amplitude row of
+// an event read as the real vector x_e = (Re A_0, Im A_0, Re A_1, ...) of length
+// D = 2W (the interleaved complex128 row, no copy), the real symmetric matrix
+//   C = sum_e wt_e x_e x_e^T                      (D x D)
+// holds every product:
+//   Re I_ij = C[2i][2j] + C[2i+1][2j+1],   Im I_ij = C[2i+1][2j] - C[2i][2j+1].
+// C is computed as a rank-k update on v_mfma_f64_16x16x4_f64: 16x16 output
+// tiles of the upper triangle (tile row <= tile column), k = events.
+// A persistent grid of one 256-thread workgroup per CU walks 16-event chunks
+// (grid-stride); each chunk is staged once through LDS and every wave issues
+// the MFMAs of its tiles from it.  Each workgroup writes its tile sums to a
+// partial row; partial rows are summed in fixed order (integral_reduce), and
+// the host assembles the complex matrix.
+#include "internal.h"
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace rpwa_amd {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIntThreads = 256;
+constexpr int kIntWaves = 4;
+constexpr int kChunk = 16;       // events per staged chunk (4 MFMA k-steps)
+constexpr int kMaxTilesPerWave = 24;
+
+struct IntArgs {
+	const double* x;       // [M][D] row-major (interleaved complex rows)
+	const double* w;       // importance weights [M] or null (weight = 1/w)
+	uint64_t M;
+	uint32_t D;            // 2W
+	uint32_t Dp;           // D padded to a multiple of 16
+	uint32_t ldsStride;    // doubles per LDS row
+	uint32_t nT;           // Dp / 16
+	uint32_t nTiles;       // nT (nT + 1) / 2
+	uint32_t tileBase;     // first tile of this pass
+	uint32_t tileEnd;      // one past the last tile of this pass
+	double* partials;      // [nWg][nTiles][256]
+	double* wsum;          // [nWg] sum of weights (1/w), or null
+	int accumulate;        // add to the existing partials
+};
+
+__device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, uint32_t& J)
+{
+	// tiles enumerated row by row over the upper triangle
+	uint32_t i = 0, rem = t;
+	while (rem >= nT - i) {
+		rem -= nT - i;
+		++i;
+	}
+	I = i;
+	J = i + rem;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
+{
+	extern __shared__ double xs[];   // [kChunk][ldsStride] + weights[kChunk]
+	double* wts = xs + (size_t)kChunk * a.ldsStride;
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wid = tid >> 6;
+	const int li = lane & 15;   // row/column within the tile
+	const int lk = lane >> 4;   // k index within a 4-event step
+
+	// tiles of this wave: t = wid, wid + 4, ...
+	d4 acc[NT];
+	uint32_t tI[NT], tJ[NT];
+	int nMine = 0;
+	for (int q = 0; q < NT; ++q) {
+		const uint32_t t = a.tileBase + (uint32_t)wid + (uint32_t)q * kIntWaves;
+		uint32_t I = 0, J = 0;
+		if (t < a.tileEnd) {
+			tile_of(t, a.nT, I, J);
+			nMine = q + 1;
+		}
+		tI[q] = I;
+		tJ[q] = J;
+		acc[q] = d4{0., 0., 0., 0.};
+	}
+	double wacc = 0.;
+
+	const uint64_t nChunks = (a.M + kChunk - 1) / kChunk;
+	for (uint64_t c = blockIdx.x; c < nChunks; c += gridDim.x) {
+		const uint64_t e0 = c * kChunk;
+		__syncthreads();
+		// stage the chunk: rows e0..e0+15, columns 0..Dp-1 (zero padded)
+		for (uint32_t idx = tid; idx < (uint32_t)kChunk * a.Dp; idx += kIntThreads) {
+			const uint32_t r = idx / a.Dp;
+			const uint32_t col = idx % a.Dp;
+			const uint64_t e = e0 + r;
+			double v = 0.;
+			if (e < a.M && col < a.D)
+				v = a.x[e * a.D + col];
+			xs[(size_t)r * a.ldsStride + col] = v;
+		}
+		if (tid < kChunk) {
+			const uint64_t e = e0 + tid;
+			double wt = 0.;
+			if (e < a.M)
+				wt = a.w ? 1. / a.w[e] : 1.;
+			wts[tid] = wt;
+			wacc += wt;
+		}
+		__syncthreads();
+		for (int ks = 0; ks < kChunk / 4; ++ks) {
+			const int row = ks * 4 + lk;
+			const double wt = wts[row];
+			const double* xr = xs + (size_t)row * a.ldsStride;
+			for (int q = 0; q < NT; ++q) {
+				if (q >= nMine)
+					break;
+				const double av = xr[tI[q] * 16 + li] * wt;
+				const double bv = xr[tJ[q] * 16 + li];
+				acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+			}
+		}
+	}
+
+	// v_mfma_f64_16x16x4_f64 result layout (checked on gfx950):
+	// D[i][j] sits in lane j + 16 * (i % 4), register i / 4
+	double* out = a.partials + (size_t)blockIdx.x * a.nTiles * 256;
+	for (int q = 0; q < NT; ++q) {
+		if (q >= nMine)
+			break;
+		const uint32_t t = a.tileBase + (uint32_t)wid + (uint32_t)q * kIntWaves;
+		double* o = out + (size_t)t * 256;
+		for (int v = 0; v < 4; ++v) {
+			const int i = lk + 4 * v;
+			const int j = li;
+			double val = acc[q][v];
+			if (a.accumulate)
+				val += o[i * 16 + j];
+			o[i * 16 + j] = val;
+		}
+	}
+	__shared__ double wsh[kChunk];
+	if (tid < kChunk)
+		wsh[tid] = wacc;
+	__syncthreads();
+	if (a.wsum && a.tileBase == 0 && tid == 0) {
+		double s = 0.;
+		for (int r = 0; r < kChunk; ++r)
+			s += wsh[r];
+		a.wsum[blockIdx.x] = a.accumulate ? a.wsum[blockIdx.x] + s : s;
+	}
+}
+
+// fixed-order column sums of a [nRows][nCols] array
+constexpr int kColElems = 64;
+constexpr int kColChunks = 16;
+__global__ __launch_bounds__(kColElems* kColChunks) void column_sum_kernel(
+    const double* __restrict__ in, int nRows, uint32_t nCols, double* __restrict__ out)
+{
+	__shared__ double sh[kColChunks][kColElems];
+	const int el = threadIdx.x % kColElems;
+	const int ch = threadIdx.x / kColElems;
+	const uint32_t col = blockIdx.x * kColElems + el;
+	const int r0 = (int)((int64_t)nRows * ch / kColChunks);
+	const int r1 = (int)((int64_t)nRows * (ch + 1) / kColChunks);
+	double s = 0.;
+	if (col < nCols)
+		for (int r = r0; r < r1; ++r)
+			s += in[(size_t)r * nCols + col];
+	sh[ch][el] = s;
+	__syncthreads();
+	if (ch == 0 && col < nCols) {
+		double t = 0.;
+		for (int k = 0; k < kColChunks; ++k)
+			t += sh[k][el];
+		out[col] = t;
+	}
+}
+
+}  // namespace
+
+struct IntegralPlan {
+	uint32_t D, Dp, nT, nTiles, ldsStride, tilesPerPass;
+	int nWg;
+	size_t lds;
+};
+
+static int integral_plan(int device, uint32_t nWaves, IntegralPlan& p)
+{
+	p.D = 2 * nWaves;
+	p.Dp = (p.D + 15) / 16 * 16;
+	p.nT = p.Dp / 16;
+	p.nTiles = p.nT * (p.nT + 1) / 2;
+	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
+	p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
+	p.lds = ((size_t)kChunk * p.ldsStride + kChunk) * sizeof(double);
+	int nCu = 0;
+	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
+	p.nWg = nCu > 0 ? nCu : 1;
+	return RPWA_OK;
+}
+
+typedef void (*IntKernel)(IntArgs);
+static IntKernel pick_int_kernel(uint32_t nTiles)
+{
+	const uint32_t per = (nTiles + kIntWaves - 1) / kIntWaves;
+	if (per <= 1) return integral_kernel<1>;
+	if (per <= 4) return integral_kernel<4>;
+	if (per <= 8) return integral_kernel<8>;
+	if (per <= 12) return integral_kernel<12>;
+	if (per <= 17) return integral_kernel<17>;
+	return integral_kernel<24>;
+}
+
+// assemble the complex [W][W] sums from the reduced upper-triangle tiles
+static void assemble(const std::vector<double>& C, const IntegralPlan& p, uint32_t W, double* out)
+{
+	auto tileIndex = [&](uint32_t I, uint32_t J) {
+		// row-by-row enumeration of the upper triangle
+		return I * p.nT - I * (I - 1) / 2 + (J - I);
+	};
+	auto getC = [&](uint32_t r, uint32_t c) {
+		uint32_t I = r / 16, J = c / 16;
+		if (I > J) {
+			std::swap(r, c);
+			std::swap(I, J);
+		}
+		return C[(size_t)tileIndex(I, J) * 256 + (r % 16) * 16 + (c % 16)];
+	};
+	for (uint32_t i = 0; i < W; ++i)
+		for (uint32_t j = 0; j < W; ++j) {
+			const double re = getC(2 * i, 2 * j) + getC(2 * i + 1, 2 * j + 1);
+			const double im = getC(2 * i + 1, 2 * j) - getC(2 * i, 2 * j + 1);
+			out[2 * ((size_t)i * W + j)] = re;
+			out[2 * ((size_t)i * W + j) + 1] = im;
+		}
+}
+
+static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W,
+                        const double* d_w, double* outSums, const double* h_amps,
+                        const double* h_w)
+{
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	IntegralPlan p;
+	int st = integral_plan(device, W, p);
+	if (st != RPWA_OK)
+		return st;
+	hipStream_t s;
+	RPWA_HIP_CHECK(hipStreamCreate(&s));
+	double *d_part = nullptr, *d_wsum = nullptr, *d_red = nullptr;
+	double* d_stage[2] = {nullptr, nullptr};
+	double* d_wstage[2] = {nullptr, nullptr};
+	const size_t partN = (size_t)p.nWg * p.nTiles * 256;
+	const bool weighted = (d_w != nullptr) || (h_w != nullptr);
+	std::vector<double> C((size_t)p.nTiles * 256);
+	std::vector<double> wsums(p.nWg);
+	auto fail = [&](int code) {
+		hipFree(d_part); hipFree(d_wsum); hipFree(d_red);
+		hipFree(d_stage[0]); hipFree(d_stage[1]); hipFree(d_wstage[0]); hipFree(d_wstage[1]);
+		hipStreamDestroy(s);
+		return code;
+	};
+#define INT_CHECK(expr)                                                               \
+	do {                                                                              \
+		hipError_t e_ = (expr);                                                       \
+		if (e_ != hipSuccess) {                                                       \
+			set_error(std::string(#expr) + ": " + hipGetErrorString(e_));             \
+			return fail(RPWA_ERR_HIP);                                                \
+		}                                                                             \
+	} while (0)
+	INT_CHECK(hipMalloc(&d_part, partN * sizeof(double)));
+	INT_CHECK(hipMalloc(&d_wsum, p.nWg * sizeof(double)));
+	INT_CHECK(hipMalloc(&d_red, (size_t)p.nTiles * 256 * sizeof(double)));
+	IntKernel k = pick_int_kernel(p.tilesPerPass);
+	// every pass re-reads the amplitudes for its own set of output tiles
+	auto launch_passes = [&](IntArgs aa) -> hipError_t {
+		for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
+			aa.tileBase = tb;
+			aa.tileEnd = std::min(p.nTiles, tb + p.tilesPerPass);
+			hipLaunchKernelGGL(k, dim3(p.nWg), dim3(kIntThreads), p.lds, s, aa);
+			hipError_t e = hipGetLastError();
+			if (e != hipSuccess)
+				return e;
+		}
+		return hipSuccess;
+	};
+	IntArgs a;
+	a.D = p.D; a.Dp = p.Dp; a.ldsStride = p.ldsStride; a.nT = p.nT; a.nTiles = p.nTiles;
+	a.partials = d_part; a.wsum = d_wsum;
+	if (d_amps) {
+		a.x = d_amps; a.w = d_w; a.M = M; a.accumulate = 0;
+		INT_CHECK(launch_passes(a));
+	} else {
+		// stream the host array through two device buffers
+		const uint64_t chunkEv = std::max<uint64_t>(1, (256ull << 20) / (16ull * W));
+		for (int b = 0; b < 2; ++b) {
+			INT_CHECK(hipMalloc(&d_stage[b], chunkEv * 16ull * W));
+			if (weighted)
+				INT_CHECK(hipMalloc(&d_wstage[b], chunkEv * sizeof(double)));
+		}
+		hipEvent_t done[2];
+		INT_CHECK(hipEventCreate(&done[0]));
+		INT_CHECK(hipEventCreate(&done[1]));
+		bool first = true;
+		int b = 0;
+		for (uint64_t off = 0; off < M; off += chunkEv, b ^= 1) {
+			const uint64_t n = std::min<uint64_t>(chunkEv, M - off);
+			if (!first)
+				INT_CHECK(hipEventSynchronize(done[b]));
+			INT_CHECK(hipMemcpyAsync(d_stage[b], h_amps + 2 * off * W, n * 16ull * W,
+			                         hipMemcpyHostToDevice, s));
+			if (weighted)
+				INT_CHECK(hipMemcpyAsync(d_wstage[b], h_w + off, n * sizeof(double),
+				                         hipMemcpyHostToDevice, s));
+			a.x = d_stage[b]; a.w = weighted ? d_wstage[b] : nullptr; a.M = n;
+			a.accumulate = first ? 0 : 1;
+			INT_CHECK(launch_passes(a));
+			INT_CHECK(hipEventRecord(done[b], s));
+			first = false;
+		}
+		INT_CHECK(hipStreamSynchronize(s));
+		hipEventDestroy(done[0]);
+		hipEventDestroy(done[1]);
+	}
+	const int nb = (int)((p.nTiles * 256 + kColElems - 1) / kColElems);
+	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, s, d_part,
+	                   p.nWg, p.nTiles * 256, d_red);
+	INT_CHECK(hipGetLastError());
+	INT_CHECK(hipMemcpyAsync(C.data(), d_red, C.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	INT_CHECK(hipMemcpyAsync(wsums.data(), d_wsum, wsums.size() * sizeof(double),
+	                         hipMemcpyDeviceToHost, s));
+	INT_CHECK(hipStreamSynchronize(s));
+#undef INT_CHECK
+	assemble(C, p, W, outSums);
+	if (weighted) {
+		double wtot = 0.;
+		for (double v : wsums)
+			wtot += v;
+		const double weightNorm = wtot / (double)M;
+		const double inv = 1 / weightNorm;
+		for (size_t i = 0; i < (size_t)2 * W * W; ++i)
+			outSums[i] *= inv;
+	}
+	fail(RPWA_OK);
+	return RPWA_OK;
+}
+
+}  // namespace rpwa_amd
+
+extern "C" int rpwa_integral(int device, const double* amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                             const double* weights, double* outSums)
+{
+	rpwa_amd::clear_error();
+	if (!amps || !outSums || nmbEvents == 0 || nmbWaves == 0) {
+		rpwa_amd::set_error("rpwa_integral: null pointer or empty input");
+		return RPWA_ERR_INVALID;
+	}
+	try {
+		return rpwa_amd::integral_run(device, nullptr, nmbEvents, nmbWaves, nullptr, outSums, amps,
+		                              weights);
+	} catch (const std::exception& ex) {
+		rpwa_amd::set_error(ex.what());
+		return RPWA_ERR_NOMEM;
+	}
+}
+
+extern "C" int rpwa_integral_device(int device, const double* d_amps, uint64_t nmbEvents,
+                                    uint32_t nmbWaves, const double* d_weights, double* outSums)
+{
+	rpwa_amd::clear_error();
+	if (!d_amps || !outSums || nmbEvents == 0 || nmbWaves == 0) {
+		rpwa_amd::set_error("rpwa_integral_device: null pointer or empty input");
+		return RPWA_ERR_INVALID;
+	}
+	try {
+		return rpwa_amd::integral_run(device, d_amps, nmbEvents, nmbWaves, d_weights, outSums,
+		                              nullptr, nullptr);
+	} catch (const std::exception& ex) {
+		rpwa_amd::set_error(ex.what());
+		return RPWA_ERR_NOMEM;
+	}
+}

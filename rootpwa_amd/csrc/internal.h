@@ -1,0 +1,106 @@
+// internal.h -- shared declarations of the gfx950 backend (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+
+#include "rootpwa_amd.h"
+
+namespace rpwa_amd {
+
+// error reporting: every C entry point catches and converts to a status code
+void set_error(const std::string& msg);
+void clear_error();
+
+#define RPWA_HIP_CHECK(expr)                                                            \
+	do {                                                                                \
+		hipError_t e_ = (expr);                                                         \
+		if (e_ != hipSuccess) {                                                         \
+			::rpwa_amd::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+			return RPWA_ERR_HIP;                                                        \
+		}                                                                               \
+	} while (0)
+
+// Highest rank held in registers by the event kernels.
+constexpr int kMaxRank = 4;
+
+// Device layout of a likelihood context's amplitudes: one row per event; the
+// row holds the refl -1 waves, then the refl +1 waves.  In mixed precision
+// (complex64) each sector is padded to an even number of waves so that a lane
+// can load two complex64 values of one sector with one 16-byte load.
+struct AmpLayout {
+	uint64_t nEvents = 0;
+	uint32_t nW[2] = {0, 0};
+	uint32_t wmax = 0;
+	int precision = RPWA_PREC_FP64;
+	uint32_t perSlot = 1;      // complex values per lane slot (1: fp64, 2: complex64)
+	uint32_t secOff[2] = {0, 0};  // offset of each sector in a row [complex units]
+	uint32_t rowLen = 0;       // row length [complex units]
+	size_t bytes() const { return (size_t)nEvents * rowLen * (precision == RPWA_PREC_FP64 ? 16 : 8); }
+};
+
+// Per-launch description of the event tiling.
+struct TileShape {
+	int lanes = 16;        // lanes cooperating on one event
+	int slots = 1;         // slots per lane (each perSlot complex values)
+	int kmax = 2;          // register capacity (template) >= slots
+	int slot1 = 0;         // first slot of sector 1
+};
+
+struct EventArgs {
+	const void* amps;
+	uint64_t nEvents;
+	uint32_t rowLen;
+	uint32_t secOff0, secOff1;
+	uint32_t nW0, nW1;
+	uint32_t wmax;
+	int rank;
+	int slots;
+	int slot1;
+	const double2* prod;   // device [rank][2][wmax] complex
+	double flat2;
+	double* partials;      // [nWg][nElem]
+	uint32_t nElem;
+};
+
+// Number of doubles in a workgroup partial: lnL sum, lnL compensation,
+// sum of weights -2/sigma, then [rank][2][wmax] complex derivatives.
+inline uint32_t partial_elems(int rank, uint32_t wmax) { return 3u + 2u * (uint32_t)rank * 2u * wmax; }
+
+// launchers (lik_kernels.hip)
+hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
+                               bool grad, int nWg, hipStream_t s);
+int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, bool grad, int* wgPerCu);
+hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
+                               double* out, hipStream_t s);
+hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int srcLayout,
+                                 const double* d_scale, void* d_dst, hipStream_t s);
+
+}  // namespace rpwa_amd
+
+struct rpwa_lik_ctx {
+	std::atomic<int> refs{1};
+	int device = 0;
+	int precision = RPWA_PREC_FP64;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, evMid = nullptr, ev1 = nullptr;
+	rpwa_amd::AmpLayout lay;
+	rpwa_amd::TileShape tile;
+	bool loaded = false;
+	int nCu = 0;
+	int nWg = 0;
+	void* d_amps = nullptr;
+	double* d_prod = nullptr;      // [kMaxRank][2][wmax] complex
+	double* d_partials = nullptr;  // [nWg][partial_elems]
+	double* d_out = nullptr;       // reduced results
+	double* h_prod = nullptr;      // pinned staging
+	double* h_out = nullptr;       // pinned staging
+	size_t partialCap = 0;
+	double kernelTime = 0;
+	double eventTime = 0;
+	double reduceTime = 0;
+	uint64_t nEvals = 0;
+};

@@ -1,0 +1,655 @@
+// pwalik_capi.cc -- C ABI over rpwa::pwaLikelihood / rpwa::ampIntegralMatrix
+// (declared in include/rootpwa_amd_pwalik.h).
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "rootpwa_amd_pwalik.h"
+#include "rpwa/pwaLikelihood.h"
+
+namespace rpwa_amd {
+void set_error(const std::string& msg);
+void clear_error();
+}  // namespace rpwa_amd
+
+using rpwa_amd::clear_error;
+using rpwa_amd::set_error;
+
+struct rpwa_intmat {
+	rpwa::ampIntegralMatrix m;
+};
+
+struct rpwa_pwalik {
+	std::unique_ptr<rpwa::pwaLikelihood<std::complex<double> > > d;
+	std::unique_ptr<rpwa::pwaLikelihood<std::complex<float> > > f;
+};
+
+#define GUARD_BEGIN                \
+	clear_error();                 \
+	try {
+#define GUARD_END                                        \
+	}                                                    \
+	catch (const std::exception& ex) {                   \
+		set_error(ex.what());                            \
+		return RPWA_ERR_INVALID;                         \
+	}                                                    \
+	catch (...) {                                        \
+		set_error("unknown exception");                  \
+		return RPWA_ERR_INVALID;                         \
+	}
+
+// run `body` with `lik` bound to whichever instantiation the handle holds
+#define WITH_LIK(L, ...)               \
+	do {                               \
+		if ((L)->d) {                  \
+			auto& lik = *(L)->d;       \
+			__VA_ARGS__;               \
+		} else {                       \
+			auto& lik = *(L)->f;       \
+			__VA_ARGS__;               \
+		}                              \
+	} while (0)
+
+static int need(const void* p, const char* fn)
+{
+	if (!p) {
+		set_error(std::string(fn) + ": null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	return RPWA_OK;
+}
+
+static int fail(const char* fn, const char* what)
+{
+	set_error(std::string(fn) + ": " + what);
+	return RPWA_ERR_INVALID;
+}
+
+// ---- ampIntegralMatrix ------------------------------------------------------
+extern "C" int rpwa_intmat_create(rpwa_intmat** m)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_create"))
+		return st;
+	*m = new rpwa_intmat();
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_destroy(rpwa_intmat* m)
+{
+	clear_error();
+	delete m;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_intmat_set_wave_names(rpwa_intmat* m, uint32_t nmbWaves, const char* const* names)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_set_wave_names"))
+		return st;
+	if (int st = need(names, "rpwa_intmat_set_wave_names"))
+		return st;
+	std::vector<std::string> n(names, names + nmbWaves);
+	if (not m->m.setWaveNames(n))
+		return fail("rpwa_intmat_set_wave_names", "wave names rejected (duplicates or a different set)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_nmb_waves(const rpwa_intmat* m, uint32_t* nmbWaves)
+{
+	GUARD_BEGIN
+	if (!m || !nmbWaves)
+		return fail("rpwa_intmat_nmb_waves", "null pointer");
+	*nmbWaves = m->m.nmbWaves();
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_wave_name(const rpwa_intmat* m, uint32_t i, char* name, int nameLen)
+{
+	GUARD_BEGIN
+	if (!m || !name || nameLen <= 0)
+		return fail("rpwa_intmat_wave_name", "null pointer");
+	if (i >= m->m.nmbWaves())
+		return fail("rpwa_intmat_wave_name", "index out of range");
+	std::strncpy(name, m->m.waveName(i).c_str(), (size_t)nameLen - 1);
+	name[nameLen - 1] = 0;
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_set_sums(rpwa_intmat* m, const double* sums, uint64_t nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_set_sums"))
+		return st;
+	if (int st = need(sums, "rpwa_intmat_set_sums"))
+		return st;
+	std::vector<std::complex<double> >& v = m->m.matrix();
+	std::memcpy(v.data(), sums, v.size() * sizeof(std::complex<double>));
+	m->m.setNmbEvents(nmbEvents);
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_get_sums(const rpwa_intmat* m, double* sums, uint64_t* nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_get_sums"))
+		return st;
+	if (sums)
+		std::memcpy(sums, m->m.matrix().data(), m->m.matrix().size() * sizeof(std::complex<double>));
+	if (nmbEvents)
+		*nmbEvents = m->m.nmbEvents();
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_set_nmb_events(rpwa_intmat* m, uint64_t nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_set_nmb_events"))
+		return st;
+	m->m.setNmbEvents(nmbEvents);
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_element(const rpwa_intmat* m, uint32_t i, uint32_t j, double* re, double* im)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_element"))
+		return st;
+	const std::complex<double> v = m->m.element(i, j);
+	if (re)
+		*re = v.real();
+	if (im)
+		*im = v.imag();
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_integrate(rpwa_intmat* m, const double* amps, uint64_t nmbEvents, const double* weights,
+                                     int device, int onDevice)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_integrate"))
+		return st;
+	const bool ok = onDevice ? m->m.integrateDevice(amps, nmbEvents, weights, device)
+	                         : m->m.integrate(reinterpret_cast<const std::complex<double>*>(amps), nmbEvents,
+	                                          weights, device);
+	if (not ok) {
+		const std::string prev = rpwa_last_error();
+		set_error("rpwa_intmat_integrate failed" + (prev.empty() ? std::string() : ": " + prev));
+		return RPWA_ERR_INVALID;
+	}
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_add(rpwa_intmat* m, const rpwa_intmat* other)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_add"))
+		return st;
+	if (int st = need(other, "rpwa_intmat_add"))
+		return st;
+	m->m += other->m;
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_renormalize(rpwa_intmat* m, uint64_t nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_renormalize"))
+		return st;
+	m->m.renormalize(nmbEvents);
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_write_ascii(const rpwa_intmat* m, const char* fileName)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_write_ascii"))
+		return st;
+	if (not m->m.writeAscii(std::string(fileName ? fileName : "")))
+		return fail("rpwa_intmat_write_ascii", "cannot write file");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_intmat_read_ascii(rpwa_intmat* m, const char* fileName)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_read_ascii"))
+		return st;
+	if (not m->m.readAscii(std::string(fileName ? fileName : "")))
+		return fail("rpwa_intmat_read_ascii", "cannot read file or malformed content");
+	return RPWA_OK;
+	GUARD_END
+}
+
+// ---- pwaLikelihood ------------------------------------------------------------
+extern "C" int rpwa_pwalik_create(rpwa_pwalik** L, int precision)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_create"))
+		return st;
+	rpwa_pwalik* h = new rpwa_pwalik();
+	if (precision == RPWA_PREC_MIXED)
+		h->f.reset(new rpwa::pwaLikelihood<std::complex<float> >());
+	else if (precision == RPWA_PREC_FP64)
+		h->d.reset(new rpwa::pwaLikelihood<std::complex<double> >());
+	else {
+		delete h;
+		return fail("rpwa_pwalik_create", "unknown precision");
+	}
+	*L = h;
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_clone(const rpwa_pwalik* L, rpwa_pwalik** out)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_clone"))
+		return st;
+	if (int st = need(out, "rpwa_pwalik_clone"))
+		return st;
+	rpwa_pwalik* h = new rpwa_pwalik();
+	if (L->d)
+		h->d.reset(L->d->Clone());
+	else
+		h->f.reset(L->f->Clone());
+	*out = h;
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_destroy(rpwa_pwalik* L)
+{
+	clear_error();
+	delete L;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_pwalik_set_device(rpwa_pwalik* L, int device)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_set_device"))
+		return st;
+	WITH_LIK(L, lik.setGpuDevice(device));
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_init(rpwa_pwalik* L, uint32_t nmbWaves, const char* const* names,
+                                const int* reflectivities, const double* thresholds, uint32_t rank,
+                                double massBinCenter)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_init"))
+		return st;
+	if (!names || !reflectivities)
+		return fail("rpwa_pwalik_init", "null wave list");
+	bool ok = false;
+	WITH_LIK(L, {
+		typedef typename std::remove_reference<decltype(lik)>::type likT;
+		std::vector<typename likT::waveDescThresType> waves;
+		for (uint32_t i = 0; i < nmbWaves; ++i)
+			waves.push_back({std::string(names[i]), reflectivities[i], thresholds ? thresholds[i] : 0.});
+		ok = lik.init(waves, rank, massBinCenter);
+	});
+	if (!ok)
+		return fail("rpwa_pwalik_init", "invalid wave list or rank");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_use_normalized_amps(rpwa_pwalik* L, int flag)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_use_normalized_amps"))
+		return st;
+	WITH_LIK(L, lik.useNormalizedAmps(flag != 0));
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_set_prior(rpwa_pwalik* L, int halfCauchy, double cauchyWidth)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_set_prior"))
+		return st;
+	WITH_LIK(L, {
+		typedef typename std::remove_reference<decltype(lik)>::type likT;
+		lik.setPriorType(halfCauchy ? likT::HALF_CAUCHY : likT::FLAT);
+		lik.setCauchyWidth(cauchyWidth);
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_add_norm_integral(rpwa_pwalik* L, const rpwa_intmat* m)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_add_norm_integral"))
+		return st;
+	if (int st = need(m, "rpwa_pwalik_add_norm_integral"))
+		return st;
+	bool ok = false;
+	WITH_LIK(L, ok = lik.addNormIntegral(m->m));
+	if (!ok)
+		return fail("rpwa_pwalik_add_norm_integral", "rejected (call order or wave set)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_add_acc_integral(rpwa_pwalik* L, rpwa_intmat* m, uint32_t accEventsOverride)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_add_acc_integral"))
+		return st;
+	if (int st = need(m, "rpwa_pwalik_add_acc_integral"))
+		return st;
+	bool ok = false;
+	WITH_LIK(L, ok = lik.addAccIntegral(m->m, accEventsOverride));
+	if (!ok)
+		return fail("rpwa_pwalik_add_acc_integral", "rejected (call order or wave set)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_add_amplitude(rpwa_pwalik* L, const char* waveName, const double* amps,
+                                         uint64_t nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_add_amplitude"))
+		return st;
+	if (!waveName || !amps)
+		return fail("rpwa_pwalik_add_amplitude", "null input");
+	bool ok = false;
+	WITH_LIK(L, ok = lik.addAmplitude(waveName, reinterpret_cast<const std::complex<double>*>(amps), nmbEvents));
+	if (!ok)
+		return fail("rpwa_pwalik_add_amplitude", "rejected (call order, unknown wave, event count or zero integral)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_add_amplitudes_device(rpwa_pwalik* L, const double* d_amps, uint64_t nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_add_amplitudes_device"))
+		return st;
+	bool ok = false;
+	WITH_LIK(L, ok = lik.addAmplitudesDevice(d_amps, nmbEvents));
+	if (!ok)
+		return fail("rpwa_pwalik_add_amplitudes_device", "rejected (call order, empty input or zero integral)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_finish_init(rpwa_pwalik* L)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_finish_init"))
+		return st;
+	bool ok = false;
+	WITH_LIK(L, ok = lik.finishInit());
+	if (!ok) {
+		const std::string prev = rpwa_last_error();
+		return fail("rpwa_pwalik_finish_init", prev.empty() ? "missing amplitudes" : prev.c_str());
+	}
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_nmb_pars(const rpwa_pwalik* L, uint32_t* nmbPars, uint32_t* nmbParsFixed)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_nmb_pars"))
+		return st;
+	WITH_LIK(L, {
+		if (nmbPars)
+			*nmbPars = lik.nmbPars();
+		if (nmbParsFixed)
+			*nmbParsFixed = lik.nmbParsFixed();
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_nmb_events(const rpwa_pwalik* L, uint64_t* nmbEvents)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_nmb_events"))
+		return st;
+	WITH_LIK(L, *nmbEvents = lik.nmbEventsLong());
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_nmb_waves(const rpwa_pwalik* L, int reflectivity, uint32_t* nmbWaves)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_nmb_waves"))
+		return st;
+	WITH_LIK(L, *nmbWaves = lik.nmbWaves(reflectivity));
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_parameter(const rpwa_pwalik* L, uint32_t i, char* name, int nameLen, uint32_t* rank,
+                                     int* fixed, int* realPart)
+{
+	GUARD_BEGIN
+	if (int st = need(L, "rpwa_pwalik_parameter"))
+		return st;
+	int st = RPWA_OK;
+	WITH_LIK(L, {
+		if (i >= lik.nmbPars()) {
+			st = fail("rpwa_pwalik_parameter", "index out of range");
+		} else {
+			const auto& p = lik.parameter(i);
+			if (name && nameLen > 0) {
+				std::strncpy(name, p.parName().c_str(), (size_t)nameLen - 1);
+				name[nameLen - 1] = 0;
+			}
+			if (rank)
+				*rank = p.rank();
+			if (fixed)
+				*fixed = p.fixed() ? 1 : 0;
+			if (realPart)
+				*realPart = p.realPart() ? 1 : 0;
+		}
+	});
+	return st;
+	GUARD_END
+}
+
+#define EVAL_GUARD_END                                   \
+	}                                                    \
+	catch (const std::logic_error& ex) {                 \
+		set_error(ex.what());                            \
+		return RPWA_ERR_STATE;                           \
+	}                                                    \
+	catch (const std::exception& ex) {                   \
+		set_error(ex.what());                            \
+		return RPWA_ERR_HIP;                             \
+	}
+
+extern "C" int rpwa_pwalik_doeval(const rpwa_pwalik* L, const double* par, double* value)
+{
+	GUARD_BEGIN
+	if (!L || !par || !value)
+		return fail("rpwa_pwalik_doeval", "null pointer");
+	WITH_LIK(L, *value = lik.DoEval(par));
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_gradient(const rpwa_pwalik* L, const double* par, double* gradient)
+{
+	GUARD_BEGIN
+	if (!L || !par || !gradient)
+		return fail("rpwa_pwalik_gradient", "null pointer");
+	WITH_LIK(L, lik.Gradient(par, gradient));
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_fdf(const rpwa_pwalik* L, const double* par, double* value, double* gradient)
+{
+	GUARD_BEGIN
+	if (!L || !par || !value || !gradient)
+		return fail("rpwa_pwalik_fdf", "null pointer");
+	WITH_LIK(L, lik.FdF(par, *value, gradient));
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_doderivative(const rpwa_pwalik* L, const double* par, uint32_t k, double* value)
+{
+	GUARD_BEGIN
+	if (!L || !par || !value)
+		return fail("rpwa_pwalik_doderivative", "null pointer");
+	int st = RPWA_OK;
+	WITH_LIK(L, {
+		if (k >= lik.nmbPars())
+			st = fail("rpwa_pwalik_doderivative", "index out of range");
+		else
+			*value = lik.DoDerivative(par, k);
+	});
+	return st;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_hessian(const rpwa_pwalik* L, const double* par, double* hessian)
+{
+	GUARD_BEGIN
+	if (!L || !par || !hessian)
+		return fail("rpwa_pwalik_hessian", "null pointer");
+	WITH_LIK(L, {
+		const rpwa::denseMatrix H = lik.Hessian(par);
+		std::memcpy(hessian, H.data.data(), H.data.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_covariance(const rpwa_pwalik* L, const double* par, double* covariance)
+{
+	GUARD_BEGIN
+	if (!L || !par || !covariance)
+		return fail("rpwa_pwalik_covariance", "null pointer");
+	WITH_LIK(L, {
+		const rpwa::denseMatrix C = lik.CovarianceMatrix(par);
+		std::memcpy(covariance, C.data.data(), C.data.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_correct_param_signs(const rpwa_pwalik* L, const double* par, double* out)
+{
+	GUARD_BEGIN
+	if (!L || !par || !out)
+		return fail("rpwa_pwalik_correct_param_signs", "null pointer");
+	WITH_LIK(L, {
+		const std::vector<double> v = lik.CorrectParamSigns(par);
+		std::memcpy(out, v.data(), v.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_copy_from_par(const rpwa_pwalik* L, const double* par, double* prodAmps, double* flat)
+{
+	GUARD_BEGIN
+	if (!L || !par || !prodAmps || !flat)
+		return fail("rpwa_pwalik_copy_from_par", "null pointer");
+	WITH_LIK(L, {
+		std::vector<std::complex<double> > v;
+		lik.copyFromParArray(par, v, *flat);
+		std::memcpy(prodAmps, v.data(), v.size() * sizeof(std::complex<double>));
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_copy_to_par(const rpwa_pwalik* L, const double* prodAmps, double flat, double* par)
+{
+	GUARD_BEGIN
+	if (!L || !par || !prodAmps)
+		return fail("rpwa_pwalik_copy_to_par", "null pointer");
+	WITH_LIK(L, {
+		const size_t n = (size_t)lik.rank() * 2 * std::max(lik.nmbWaves(-1), lik.nmbWaves(+1));
+		const std::complex<double>* p = reinterpret_cast<const std::complex<double>*>(prodAmps);
+		std::vector<std::complex<double> > v(p, p + n);
+		lik.copyToParArray(v, flat, par);
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_integral_matrices(const rpwa_pwalik* L, int withFlat, double* normMatrix,
+                                             double* accMatrix, double* phaseSpaceIntegral)
+{
+	GUARD_BEGIN
+	if (!L)
+		return fail("rpwa_pwalik_integral_matrices", "null pointer");
+	WITH_LIK(L, {
+		std::vector<std::complex<double> > n, a;
+		std::vector<double> ps;
+		lik.getIntegralMatrices(n, a, ps, withFlat != 0);
+		if (normMatrix)
+			std::memcpy(normMatrix, n.data(), n.size() * sizeof(std::complex<double>));
+		if (accMatrix)
+			std::memcpy(accMatrix, a.data(), a.size() * sizeof(std::complex<double>));
+		if (phaseSpaceIntegral)
+			std::memcpy(phaseSpaceIntegral, ps.data(), ps.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_call_info(const rpwa_pwalik* L, uint32_t calls[5], double totalTime[5])
+{
+	GUARD_BEGIN
+	if (!L)
+		return fail("rpwa_pwalik_call_info", "null pointer");
+	WITH_LIK(L, {
+		typedef typename std::remove_reference<decltype(lik)>::type likT;
+		for (int i = 0; i < 5; ++i) {
+			const auto& ci = lik.callInfo((typename likT::functionCallEnum)i);
+			if (calls)
+				calls[i] = ci.nmbCalls;
+			if (totalTime)
+				totalTime[i] = ci.totalTime;
+		}
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_gpu_context(const rpwa_pwalik* L, rpwa_lik_ctx** ctx)
+{
+	GUARD_BEGIN
+	if (!L || !ctx)
+		return fail("rpwa_pwalik_gpu_context", "null pointer");
+	WITH_LIK(L, *ctx = lik.gpuContext());
+	if (!*ctx) {
+		set_error("rpwa_pwalik_gpu_context: finishInit has not set up a GPU context");
+		return RPWA_ERR_STATE;
+	}
+	return RPWA_OK;
+	GUARD_END
+}

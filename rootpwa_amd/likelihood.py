@@ -1,0 +1,392 @@
+"""Python face of the C++ likelihood (include/rootpwa_amd_pwalik.h).
+
+Mirrors what pyRootPwa exposes for this path: the pwaLikelihood class
+(pyInterface/bindings/partialWaveFit/pwaLikelihood_py.cc:243-312), the
+ampIntegralMatrix class, and the initLikelihood setup sequence of
+pyInterface/module/_likelihood.py:4-71, with arrays instead of ROOT files.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_ip = ctypes.POINTER(ctypes.c_int)
+_cpp = ctypes.POINTER(ctypes.c_char_p)
+
+PROTOTYPES = {
+    "rpwa_intmat_create": (_i, [ctypes.POINTER(_vp)]),
+    "rpwa_intmat_destroy": (_i, [_vp]),
+    "rpwa_intmat_set_wave_names": (_i, [_vp, _u32, _cpp]),
+    "rpwa_intmat_nmb_waves": (_i, [_vp, _u32p]),
+    "rpwa_intmat_wave_name": (_i, [_vp, _u32, ctypes.c_char_p, _i]),
+    "rpwa_intmat_set_sums": (_i, [_vp, _dp, _u64]),
+    "rpwa_intmat_get_sums": (_i, [_vp, _dp, _u64p]),
+    "rpwa_intmat_set_nmb_events": (_i, [_vp, _u64]),
+    "rpwa_intmat_element": (_i, [_vp, _u32, _u32, _dp, _dp]),
+    "rpwa_intmat_integrate": (_i, [_vp, _vp, _u64, _vp, _i, _i]),
+    "rpwa_intmat_add": (_i, [_vp, _vp]),
+    "rpwa_intmat_renormalize": (_i, [_vp, _u64]),
+    "rpwa_intmat_write_ascii": (_i, [_vp, ctypes.c_char_p]),
+    "rpwa_intmat_read_ascii": (_i, [_vp, ctypes.c_char_p]),
+    "rpwa_pwalik_create": (_i, [ctypes.POINTER(_vp), _i]),
+    "rpwa_pwalik_clone": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "rpwa_pwalik_destroy": (_i, [_vp]),
+    "rpwa_pwalik_set_device": (_i, [_vp, _i]),
+    "rpwa_pwalik_init": (_i, [_vp, _u32, _cpp, _ip, _dp, _u32, ctypes.c_double]),
+    "rpwa_pwalik_use_normalized_amps": (_i, [_vp, _i]),
+    "rpwa_pwalik_set_prior": (_i, [_vp, _i, ctypes.c_double]),
+    "rpwa_pwalik_add_norm_integral": (_i, [_vp, _vp]),
+    "rpwa_pwalik_add_acc_integral": (_i, [_vp, _vp, _u32]),
+    "rpwa_pwalik_add_amplitude": (_i, [_vp, ctypes.c_char_p, _dp, _u64]),
+    "rpwa_pwalik_add_amplitudes_device": (_i, [_vp, _vp, _u64]),
+    "rpwa_pwalik_finish_init": (_i, [_vp]),
+    "rpwa_pwalik_nmb_pars": (_i, [_vp, _u32p, _u32p]),
+    "rpwa_pwalik_nmb_events": (_i, [_vp, _u64p]),
+    "rpwa_pwalik_nmb_waves": (_i, [_vp, _i, _u32p]),
+    "rpwa_pwalik_parameter": (_i, [_vp, _u32, ctypes.c_char_p, _i, _u32p, _ip, _ip]),
+    "rpwa_pwalik_doeval": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_gradient": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_fdf": (_i, [_vp, _dp, _dp, _dp]),
+    "rpwa_pwalik_doderivative": (_i, [_vp, _dp, _u32, _dp]),
+    "rpwa_pwalik_hessian": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_covariance": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_correct_param_signs": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_copy_from_par": (_i, [_vp, _dp, _dp, _dp]),
+    "rpwa_pwalik_copy_to_par": (_i, [_vp, _dp, ctypes.c_double, _dp]),
+    "rpwa_pwalik_integral_matrices": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "rpwa_pwalik_call_info": (_i, [_vp, _u32p, _dp]),
+    "rpwa_pwalik_gpu_context": (_i, [_vp, ctypes.POINTER(_vp)]),
+}
+
+_ready = False
+
+
+def _lib():
+    global _ready
+    L = capi.lib()
+    if not _ready:
+        for name, (res, args) in PROTOTYPES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _ready = True
+    return L
+
+
+def _call(name, *args):
+    capi.check(name, getattr(_lib(), name)(*args))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _names(names):
+    arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    return arr
+
+
+class ampIntegralMatrix:
+    """rpwa::ampIntegralMatrix: I_ij sums over MC events and their count."""
+
+    def __init__(self, waveNames=None):
+        self.h = _vp()
+        _call("rpwa_intmat_create", ctypes.byref(self.h))
+        if waveNames is not None:
+            self.setWaveNames(waveNames)
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                _lib().rpwa_intmat_destroy(self.h)
+        except Exception:
+            pass
+
+    def setWaveNames(self, names):
+        _call("rpwa_intmat_set_wave_names", self.h, len(names), _names(names))
+
+    def nmbWaves(self):
+        n = ctypes.c_uint32()
+        _call("rpwa_intmat_nmb_waves", self.h, ctypes.byref(n))
+        return n.value
+
+    def waveNames(self):
+        out = []
+        buf = ctypes.create_string_buffer(512)
+        for i in range(self.nmbWaves()):
+            _call("rpwa_intmat_wave_name", self.h, i, buf, 512)
+            out.append(buf.value.decode())
+        return out
+
+    def setSums(self, sums, nmbEvents):
+        s = np.ascontiguousarray(sums, dtype=np.complex128)
+        assert s.shape == (self.nmbWaves(), self.nmbWaves())
+        _call("rpwa_intmat_set_sums", self.h, s.view(np.float64).ctypes.data_as(_dp), int(nmbEvents))
+
+    def sums(self):
+        W = self.nmbWaves()
+        out = np.zeros((W, W), dtype=np.complex128)
+        n = ctypes.c_uint64()
+        _call("rpwa_intmat_get_sums", self.h, out.view(np.float64).ctypes.data_as(_dp), ctypes.byref(n))
+        return out, n.value
+
+    def nmbEvents(self):
+        return self.sums()[1]
+
+    def setNmbEvents(self, n):
+        _call("rpwa_intmat_set_nmb_events", self.h, int(n))
+
+    def element(self, i, j):
+        re, im = ctypes.c_double(), ctypes.c_double()
+        if isinstance(i, str) or isinstance(j, str):
+            names = self.waveNames()
+            i = names.index(i) if isinstance(i, str) else i
+            j = names.index(j) if isinstance(j, str) else j
+        _call("rpwa_intmat_element", self.h, int(i), int(j), ctypes.byref(re), ctypes.byref(im))
+        return complex(re.value, im.value)
+
+    def integrate(self, amps, weights=None, device=0):
+        """Host amplitudes [M][W] (complex) -> sums on the GPU."""
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self._keep = (a, w)
+        _call("rpwa_intmat_integrate", self.h, a.ctypes.data_as(_vp), a.shape[0],
+              w.ctypes.data_as(_vp) if w is not None else None, int(device), 0)
+
+    def integrateDevice(self, d_amps, nmbEvents, d_weights=None, device=0):
+        _call("rpwa_intmat_integrate", self.h, _vp(d_amps), int(nmbEvents), _vp(d_weights) if d_weights else None,
+              int(device), 1)
+
+    def __iadd__(self, other):
+        _call("rpwa_intmat_add", self.h, other.h)
+        return self
+
+    def renormalize(self, n):
+        _call("rpwa_intmat_renormalize", self.h, int(n))
+
+    def writeAscii(self, fileName):
+        _call("rpwa_intmat_write_ascii", self.h, fileName.encode())
+
+    def readAscii(self, fileName):
+        _call("rpwa_intmat_read_ascii", self.h, fileName.encode())
+
+
+class pwaLikelihood:
+    """rpwa::pwaLikelihood evaluated on the GPU (C++ object behind the C ABI)."""
+
+    FLAT = 0
+    HALF_CAUCHY = 1
+
+    def __init__(self, precision=capi.PREC_FP64, _handle=None):
+        self.h = _vp()
+        if _handle is not None:
+            self.h = _handle
+        else:
+            _call("rpwa_pwalik_create", ctypes.byref(self.h), int(precision))
+        self.precision = precision
+        self._waves = None
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                _lib().rpwa_pwalik_destroy(self.h)
+        except Exception:
+            pass
+
+    def Clone(self):
+        out = _vp()
+        _call("rpwa_pwalik_clone", self.h, ctypes.byref(out))
+        c = pwaLikelihood(self.precision, _handle=out)
+        c._waves = self._waves
+        c._rank = self._rank
+        return c
+
+    # setup ---------------------------------------------------------------
+    def setDevice(self, device):
+        _call("rpwa_pwalik_set_device", self.h, int(device))
+
+    def init(self, waves, rank=1, massBinCenter=0.0):
+        """waves: list of (name, reflectivity, threshold)."""
+        names = [w[0] for w in waves]
+        refl = (ctypes.c_int * len(waves))(*[int(w[1]) for w in waves])
+        thr = np.array([float(w[2]) if len(w) > 2 else 0.0 for w in waves])
+        _call("rpwa_pwalik_init", self.h, len(waves), _names(names), refl, _d(thr), int(rank), float(massBinCenter))
+        self._waves = list(waves)
+        self._rank = int(rank)
+
+    def useNormalizedAmps(self, flag=True):
+        _call("rpwa_pwalik_use_normalized_amps", self.h, int(bool(flag)))
+
+    def setPriorType(self, prior, cauchyWidth=0.5):
+        _call("rpwa_pwalik_set_prior", self.h, int(prior == self.HALF_CAUCHY), float(cauchyWidth))
+
+    def addNormIntegral(self, m):
+        _call("rpwa_pwalik_add_norm_integral", self.h, m.h)
+
+    def addAccIntegral(self, m, accEventsOverride=0):
+        _call("rpwa_pwalik_add_acc_integral", self.h, m.h, int(accEventsOverride))
+
+    def addAmplitude(self, waveName, amps):
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        _call("rpwa_pwalik_add_amplitude", self.h, waveName.encode(), a.view(np.float64).ctypes.data_as(_dp),
+              a.shape[0])
+
+    def addAmplitudesDevice(self, d_amps, nmbEvents):
+        _call("rpwa_pwalik_add_amplitudes_device", self.h, _vp(d_amps), int(nmbEvents))
+
+    def finishInit(self):
+        _call("rpwa_pwalik_finish_init", self.h)
+
+    # accessors -----------------------------------------------------------
+    def nmbPars(self):
+        n = ctypes.c_uint32()
+        _call("rpwa_pwalik_nmb_pars", self.h, ctypes.byref(n), None)
+        return n.value
+
+    def NDim(self):
+        return self.nmbPars()
+
+    def nmbParsFixed(self):
+        n = ctypes.c_uint32()
+        _call("rpwa_pwalik_nmb_pars", self.h, None, ctypes.byref(n))
+        return n.value
+
+    def nmbEvents(self):
+        n = ctypes.c_uint64()
+        _call("rpwa_pwalik_nmb_events", self.h, ctypes.byref(n))
+        return n.value
+
+    def nmbWaves(self, reflectivity=0):
+        n = ctypes.c_uint32()
+        _call("rpwa_pwalik_nmb_waves", self.h, int(reflectivity), ctypes.byref(n))
+        return n.value
+
+    def rank(self):
+        return self._rank
+
+    def parameter(self, i):
+        """(parName, rank, fixed, realPart)"""
+        name = ctypes.create_string_buffer(256)
+        r = ctypes.c_uint32()
+        fx = ctypes.c_int()
+        rp = ctypes.c_int()
+        _call("rpwa_pwalik_parameter", self.h, int(i), name, 256, ctypes.byref(r), ctypes.byref(fx), ctypes.byref(rp))
+        return name.value.decode(), r.value, bool(fx.value), bool(rp.value)
+
+    # evaluation ------------------------------------------------------------
+    def _par(self, par):
+        p = np.ascontiguousarray(par, dtype=np.float64)
+        assert p.shape == (self.nmbPars(),)
+        return p
+
+    def DoEval(self, par):
+        p = self._par(par)
+        v = ctypes.c_double()
+        _call("rpwa_pwalik_doeval", self.h, _d(p), ctypes.byref(v))
+        return v.value
+
+    def Gradient(self, par):
+        p = self._par(par)
+        g = np.zeros_like(p)
+        _call("rpwa_pwalik_gradient", self.h, _d(p), _d(g))
+        return g
+
+    def FdF(self, par):
+        p = self._par(par)
+        g = np.zeros_like(p)
+        v = ctypes.c_double()
+        _call("rpwa_pwalik_fdf", self.h, _d(p), ctypes.byref(v), _d(g))
+        return v.value, g
+
+    def DoDerivative(self, par, k):
+        p = self._par(par)
+        v = ctypes.c_double()
+        _call("rpwa_pwalik_doderivative", self.h, _d(p), int(k), ctypes.byref(v))
+        return v.value
+
+    def Hessian(self, par):
+        p = self._par(par)
+        H = np.zeros((p.size, p.size))
+        _call("rpwa_pwalik_hessian", self.h, _d(p), _d(H))
+        return H
+
+    def CovarianceMatrix(self, par):
+        p = self._par(par)
+        C = np.zeros((p.size, p.size))
+        _call("rpwa_pwalik_covariance", self.h, _d(p), _d(C))
+        return C
+
+    def CorrectParamSigns(self, par):
+        p = self._par(par)
+        out = np.zeros_like(p)
+        _call("rpwa_pwalik_correct_param_signs", self.h, _d(p), _d(out))
+        return out
+
+    def copyFromParArray(self, par):
+        p = self._par(par)
+        nW = (self.nmbWaves(-1), self.nmbWaves(+1))
+        prod = np.zeros((self._rank, 2, max(nW)), dtype=np.complex128)
+        flat = ctypes.c_double()
+        _call("rpwa_pwalik_copy_from_par", self.h, _d(p), prod.view(np.float64).ctypes.data_as(_dp),
+              ctypes.byref(flat))
+        return prod, flat.value
+
+    def copyToParArray(self, prod, flat):
+        prod = np.ascontiguousarray(prod, dtype=np.complex128)
+        par = np.zeros(self.nmbPars())
+        _call("rpwa_pwalik_copy_to_par", self.h, prod.view(np.float64).ctypes.data_as(_dp), float(flat), _d(par))
+        return par
+
+    def integralMatrices(self, withFlat=False):
+        n = self.nmbWaves() + (1 if withFlat else 0)
+        N = np.zeros((n, n), dtype=np.complex128)
+        A = np.zeros((n, n), dtype=np.complex128)
+        ps = np.zeros(n)
+        _call("rpwa_pwalik_integral_matrices", self.h, int(bool(withFlat)), N.view(np.float64).ctypes.data_as(_dp),
+              A.view(np.float64).ctypes.data_as(_dp), _d(ps))
+        return N, A, ps
+
+    def gpuContext(self):
+        """Borrowed capi.LikelihoodContext view of the event-sum context."""
+        h = _vp()
+        _call("rpwa_pwalik_gpu_context", self.h, ctypes.byref(h))
+        view = capi.LikelihoodContext.__new__(capi.LikelihoodContext)
+        view.h = h
+        view.precision = self.precision
+        view.nW = (self.nmbWaves(-1), self.nmbWaves(+1))
+        view.borrowed = True
+        return view
+
+    def callInfo(self):
+        calls = (ctypes.c_uint32 * 5)()
+        times = np.zeros(5)
+        _call("rpwa_pwalik_call_info", self.h, calls, _d(times))
+        return dict(zip(("FdF", "Gradient", "DoEval", "DoDerivative", "Hessian"), zip(list(calls), times.tolist())))
+
+
+def initLikelihood(waves, amplitudes, normIntegral, accIntegral=None, rank=1, massBinCenter=0.0,
+                   accEventsOverride=0, precision=capi.PREC_FP64, cauchyWidth=None, device=0):
+    """The setup sequence of pyRootPwa.initLikelihood (pyInterface/module/_likelihood.py:17-71).
+
+    waves: [(name, reflectivity, threshold)]; amplitudes: dict name -> complex
+    array of data events; normIntegral / accIntegral: ampIntegralMatrix."""
+    L = pwaLikelihood(precision)
+    L.setDevice(device)
+    L.init(waves, rank, massBinCenter)
+    if cauchyWidth is not None:
+        L.setPriorType(pwaLikelihood.HALF_CAUCHY, cauchyWidth)
+    L.addNormIntegral(normIntegral)
+    L.addAccIntegral(accIntegral if accIntegral is not None else normIntegral, accEventsOverride)
+    for w in waves:
+        L.addAmplitude(w[0], amplitudes[w[0]])
+    L.finishInit()
+    return L

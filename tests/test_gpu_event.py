@@ -1,0 +1,127 @@
+"""GPU parity of the event kernels (the cuda::likelihoodInterface replacement)
+against the oracle's event part, plus the bitwise contracts."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [((0, 16), 1), ((3, 5), 2), ((8, 80), 2), ((7, 34), 2), ((1, 1), 1), ((5, 0), 1), ((4, 9), 3)]
+
+
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda p: "W%d+%d_r%d" % (p[0][0], p[0][1], p[1]))
+def case(request):
+    nW, rank = request.param
+    ds = H.dataset(nW, rank, nData=777, nMC=3000, seed=11 + sum(nW) + rank)
+    L = H.oracle_likelihood(ds)
+    return ds, L, H.normalised_amps(L)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_event_part_matches_oracle(gpu, case, prec):
+    ds, L, amps = case
+    nW, rank = ds["nW"], ds["rank"]
+    print("case", nW, rank, "prec", prec, flush=True)
+    ctx = gpu.LikelihoodContext(0, prec)
+    ctx.load(amps, nW)
+    rng = np.random.default_rng(5)
+    prod = H.random_prod(rng, rank, nW, 0.8)
+    flat = 0.3
+    f_ref = L.event_loglik(prod, flat)
+    d_ref, df_ref = L.event_deriv(prod, flat)
+    f, d, df = ctx.fdf(prod, flat)
+    tol_f, tol_d = (1e-12, 1e-10) if prec == 0 else (1e-6, 1e-5)
+    assert abs(f - f_ref) <= tol_f * abs(f_ref)
+    assert H.rel_err(d, d_ref) <= tol_d
+    assert abs(df - df_ref) <= tol_d * abs(df_ref)
+    # value-only and gradient-only entry points agree bitwise with the fused one
+    assert ctx.log_likelihood(prod, flat) == f
+    d2, df2 = ctx.log_likelihood_deriv(prod, flat)
+    assert np.array_equal(d2, d) and df2 == df
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_sign_flip_and_determinism(gpu, case, prec):
+    ds, L, amps = case
+    nW, rank = ds["nW"], ds["rank"]
+    ctx = gpu.LikelihoodContext(0, prec)
+    ctx.load(amps, nW)
+    rng = np.random.default_rng(9)
+    prod = H.random_prod(rng, rank, nW, 1.1)
+    f0 = ctx.log_likelihood(prod, 0.2)
+    for _ in range(3):
+        assert ctx.log_likelihood(prod, 0.2) == f0
+    for r in range(rank):
+        for s in range(2):
+            p2 = prod.copy()
+            p2[r, s] *= -1
+            assert ctx.log_likelihood(p2, 0.2) == f0
+            assert ctx.log_likelihood(prod, -0.2) == f0
+
+
+def test_every_tile_shape_agrees(gpu, case):
+    ds, L, amps = case
+    nW, rank = ds["nW"], ds["rank"]
+    ctx = gpu.LikelihoodContext(0, 0)
+    ctx.load(amps, nW)
+    prod = H.random_prod(np.random.default_rng(3), rank, nW, 0.9)
+    f_ref = L.event_loglik(prod, 0.1)
+    d_ref, _ = L.event_deriv(prod, 0.1)
+    for lanes in (8, 16, 32, 64):
+        try:
+            ctx.set_tile(lanes)
+        except gpu.RpwaError as e:
+            assert e.code == gpu.RPWA_ERR_UNSUPPORTED
+            continue
+        f, d, _ = ctx.fdf(prod, 0.1)
+        assert abs(f - f_ref) <= 1e-12 * abs(f_ref), lanes
+        assert H.rel_err(d, d_ref) <= 1e-10, lanes
+
+
+def test_reshuffled_layout(gpu):
+    """The [2][Wmax][N] layout pwaLikelihood::finishInit builds for the GPU."""
+    ds = H.dataset((3, 6), 2, 500, 2000, seed=4)
+    L = H.oracle_likelihood(ds)
+    amps = H.normalised_amps(L)
+    nW = ds["nW"]
+    wmax = max(nW)
+    resh = np.zeros((2, wmax, amps.shape[0]), dtype=np.complex128)
+    resh[0, :nW[0]] = amps[:, :nW[0]].T
+    resh[1, :nW[1]] = amps[:, nW[0]:].T
+    ctx = gpu.LikelihoodContext(0, 0)
+    ctx.load(resh, nW, layout=gpu.LAYOUT_REFL_WAVE_EVT, nEvents=amps.shape[0])
+    prod = H.random_prod(np.random.default_rng(1), 2, nW)
+    assert abs(ctx.log_likelihood(prod, 0.5) - L.event_loglik(prod, 0.5)) <= 1e-12 * abs(L.event_loglik(prod, 0.5))
+
+
+def test_integral_matches_oracle(gpu):
+    rng = np.random.default_rng(2)
+    for W, M in ((3, 1000), (16, 5000), (88, 3001), (110, 700)):
+        a = rng.normal(size=(M, W)) + 1j * rng.normal(size=(M, W))
+        ref = O.integral(a)
+        got = gpu.integral(a)
+        assert H.rel_err(got, ref) <= 1e-12, W
+        w = rng.uniform(0.5, 2.0, size=M)
+        assert H.rel_err(gpu.integral(a, w), O.integral(a, w)) <= 1e-12, W
+
+
+def test_generator_matches_cpu_restatement(gpu):
+    seed, W = 77, 12
+    scales = O.gen_wave_scales(seed, W)
+    assert np.array_equal(scales, gpu.gen_wave_scales(seed, W))
+    buf = gpu.DeviceBuffer(500 * W * 16)
+    gpu.gen_amps_device(buf.address, seed, 3, 1000, 500, scales)
+    got = buf.download((500, W), np.complex128)
+    ref = O.gen_amps(seed, 3, 1000, 500, scales)
+    assert H.rel_err(got, ref) <= 1e-13
+    nW, rank = (2, 5), 2
+    prod = O.gen_prod_amps(seed, 2, rank, nW)
+    assert np.array_equal(prod, gpu.gen_prod_amps(seed, 2, rank, nW))
+    out = gpu.DeviceBuffer(300 * 7 * 16)
+    pool, smax = gpu.gen_accept_reject_device(out.address, seed, 4, 300, nW, rank, prod, scales)
+    data, idx, pool_c, smax_c = O.gen_accept_reject(seed, 4, 300, nW, rank, prod, scales)
+    assert pool == pool_c
+    assert abs(smax - smax_c) <= 1e-12 * smax_c
+    assert H.rel_err(out.download((300, 7), np.complex128), data) <= 1e-13

@@ -1,0 +1,130 @@
+"""GPU parity of the drop-in likelihood class (C++ pwaLikelihood behind the
+C ABI) against the oracle: DoEval, Gradient, FdF, DoDerivative,
+CorrectParamSigns, priors, and the bitwise contracts of the fit drivers."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def wave_list(nW):
+    waves = [("neg%02d" % i, -1, 0.0) for i in range(nW[0])] + [("pos%02d" % i, +1, 0.0) for i in range(nW[1])]
+    return waves
+
+
+def build_pair(nW, rank, seed, prior=None, precision=0, nData=900, nMC=4000):
+    from rootpwa_amd import likelihood as PL
+    ds = H.dataset(nW, rank, nData, nMC, seed)
+    Lo = H.oracle_likelihood(ds, prior=prior)
+    waves = wave_list(nW)
+    names = [w[0] for w in waves]
+    I = PL.ampIntegralMatrix(names)
+    I.setSums(ds["integral"], ds["nMC"])
+    amps = {names[k]: ds["data"][:, k] for k in range(len(names))}
+    Lg = PL.initLikelihood(waves, amps, I, rank=rank, precision=precision, cauchyWidth=prior)
+    return ds, Lo, Lg
+
+
+CASES = [((0, 16), 1, None), ((8, 80), 2, None), ((3, 5), 2, 0.5), ((7, 34), 2, None), ((4, 6), 3, 0.7)]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: "W%d+%d_r%d_%s" % (c[0][0], c[0][1], c[1],
+                                                                              "cauchy" if c[2] else "flat"))
+def pair(request):
+    nW, rank, prior = request.param
+    return build_pair(nW, rank, seed=100 + sum(nW) + rank, prior=prior)
+
+
+def test_parameter_layout(gpu, pair):
+    ds, Lo, Lg = pair
+    assert Lg.nmbPars() == Lo.nPars
+    assert Lg.nmbParsFixed() == Lo.nmb_pars_fixed()
+    assert Lg.nmbEvents() == Lo.nmb_events()
+    rng = np.random.default_rng(1)
+    par = rng.normal(size=Lo.nPars)
+    pg, fg = Lg.copyFromParArray(par)
+    po, fo = Lo.copy_from_par(par)
+    assert np.array_equal(pg, po) and fg == fo
+    assert np.array_equal(Lg.copyToParArray(pg, fg), Lo.copy_to_par(po, fo))
+
+
+def test_values_and_gradients(gpu, pair):
+    ds, Lo, Lg = pair
+    rng = np.random.default_rng(2)
+    for scale in (0.3, 1.0, 3.0):
+        par = rng.normal(size=Lo.nPars) * scale
+        f_ref, g_ref = Lo.fdf(par)
+        f, g = Lg.FdF(par)
+        print("f", f, f_ref, "rel", abs(f - f_ref) / abs(f_ref))
+        assert abs(f - f_ref) <= 1e-10 * max(abs(f_ref), 1.0)
+        assert H.rel_err(g, g_ref) <= 1e-10
+        assert abs(Lg.DoEval(par) - Lo.doeval(par)) <= 1e-10 * max(abs(f_ref), 1.0)
+        assert H.rel_err(Lg.Gradient(par), Lo.gradient(par)) <= 1e-10
+        for k in (0, Lo.nPars // 2, Lo.nPars - 1):
+            assert abs(Lg.DoDerivative(par, k) - g_ref[k]) <= 1e-10 * np.max(np.abs(g_ref))
+
+
+def test_bitwise_contracts(gpu, pair):
+    """pwaNloptFit.cc:238-244 and pwaFit.cc:295-300 compare DoEval bitwise."""
+    ds, Lo, Lg = pair
+    rng = np.random.default_rng(3)
+    par = rng.normal(size=Lo.nPars)
+    f, g = Lg.FdF(par)
+    assert Lg.DoEval(par) == f
+    assert np.array_equal(Lg.Gradient(par), g)
+    fixed = Lg.CorrectParamSigns(par)
+    assert np.array_equal(fixed, Lo.correct_param_signs(par))
+    assert Lg.DoEval(fixed) == f
+    # negative anchors flip whole blocks, bitwise invariant
+    par2 = par.copy()
+    pg, fl = Lg.copyFromParArray(par2)
+    pg[:, :, :] *= -1
+    par2 = Lg.copyToParArray(pg, -fl)
+    assert Lg.DoEval(par2) == f
+
+
+def test_clone_shares_context(gpu, pair):
+    ds, Lo, Lg = pair
+    c = Lg.Clone()
+    par = np.random.default_rng(4).normal(size=Lo.nPars)
+    assert c.DoEval(par) == Lg.DoEval(par)
+    del c
+    assert np.isfinite(Lg.DoEval(par))
+
+
+def test_mixed_precision_error(gpu):
+    ds, Lo, Lg = build_pair((8, 80), 2, seed=7, precision=1, nData=3000)
+    par = np.random.default_rng(5).normal(size=Lo.nPars)
+    f_ref, g_ref = Lo.fdf(par)
+    f, g = Lg.FdF(par)
+    print("mixed |dlnL|", abs(f - f_ref), "grad rel", H.rel_err(g, g_ref))
+    assert abs(f - f_ref) < 1e-3
+    assert H.rel_err(g, g_ref) < 1e-5
+
+
+def test_integral_matrix_class(gpu, tmp_path):
+    from rootpwa_amd import likelihood as PL
+    rng = np.random.default_rng(6)
+    a = rng.normal(size=(2000, 5)) + 1j * rng.normal(size=(2000, 5))
+    m = PL.ampIntegralMatrix(["a", "b", "c", "d", "e"])
+    m.integrate(a)
+    sums, n = m.sums()
+    assert n == 2000
+    assert H.rel_err(sums, O.integral(a)) <= 1e-12
+    assert abs(m.element(1, 2) - O.integral(a)[1, 2] / 2000) <= 1e-12 * abs(O.integral(a)[1, 2] / 2000)
+    m2 = PL.ampIntegralMatrix(["a", "b", "c", "d", "e"])
+    m2.integrate(a[:700])
+    m3 = PL.ampIntegralMatrix(["a", "b", "c", "d", "e"])
+    m3.integrate(a[700:])
+    m2 += m3
+    s2, n2 = m2.sums()
+    assert n2 == 2000 and H.rel_err(s2, sums) <= 1e-12
+    f = str(tmp_path / "int.txt")
+    m.writeAscii(f)
+    m4 = PL.ampIntegralMatrix()
+    m4.readAscii(f)
+    s4, n4 = m4.sums()
+    assert n4 == 2000 and np.array_equal(s4, sums) and m4.waveNames() == ["a", "b", "c", "d", "e"]
