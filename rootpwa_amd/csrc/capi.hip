@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -41,23 +43,28 @@ uint32_t sector_slots(uint32_t nw, int lanes, uint32_t perSlot)
 	return (nw + per - 1) / per;
 }
 
-// Tile for a given lane count; false if the row does not fit the register tiles.
+// Tile with `lanes` lanes per event: the fewest slots per lane such that the
+// two sectors, each on its own lanes, fit into the group; false if more than
+// 8 slots would be needed.
 bool make_tile(const AmpLayout& lay, int lanes, TileShape& ts)
 {
-	const uint32_t s0 = sector_slots(lay.nW[0], lanes, lay.perSlot);
-	const uint32_t s1 = sector_slots(lay.nW[1], lanes, lay.perSlot);
-	const uint32_t slots = s0 + s1;
-	if (slots == 0 || slots > 8)
-		return false;
-	ts.lanes = lanes;
-	ts.slots = (int)slots;
-	ts.slot1 = (int)s0;
-	ts.kmax = slots <= 2 ? 2 : (slots <= 4 ? 4 : 8);
-	return true;
+	for (int k = 1; k <= 8; ++k) {
+		const uint32_t l0 = sector_slots(lay.nW[0], k, lay.perSlot);
+		const uint32_t l1 = sector_slots(lay.nW[1], k, lay.perSlot);
+		if (l0 + l1 <= (uint32_t)lanes) {
+			ts.lanes = lanes;
+			ts.slots = k;
+			ts.lanes0 = (int)l0;
+			ts.lanes1 = (int)l1;
+			return true;
+		}
+	}
+	return false;
 }
 
-// Default tile: minimise the modelled VALU cycles per event (complex MACs of
-// the slots, the per-event log/division and the butterfly), rank 2 assumed.
+// Modelled default tile (used when autotuning is off): VALU instructions per
+// event of the complex MACs of the group's slots, the per-event log/division
+// and the butterflies, rank 2 assumed.
 bool choose_tile(const AmpLayout& lay, TileShape& best)
 {
 	double bestCost = 1e300;
@@ -66,8 +73,7 @@ bool choose_tile(const AmpLayout& lay, TileShape& best)
 		TileShape ts;
 		if (!make_tile(lay, lanes, ts))
 			continue;
-		const double perWave = ts.slots * (double)lay.perSlot * 2 * 8 + 40 + 8.0 * 2 * ilog2(lanes) +
-		                       4.0 * (ts.kmax - ts.slots);
+		const double perWave = ts.slots * (double)lay.perSlot * 2 * 8 + 60 + 8.0 * 3 * ilog2(lanes);
 		const double cost = perWave * lanes / 64.0;
 		if (cost < bestCost) {
 			bestCost = cost;
@@ -104,17 +110,17 @@ void free_buffers(rpwa_lik_ctx* c)
 	c->loaded = false;
 }
 
-// (Re)compute the persistent grid for the context's tile.
-int configure_grid(rpwa_lik_ctx* c)
+// (Re)compute the persistent grid of a rank's tile.
+int configure_grid(rpwa_lik_ctx* c, int rank)
 {
 	int wgPerCu = 1;
-	int st = event_kernel_occupancy(c->lay, c->tile, true, &wgPerCu);
+	int st = event_kernel_occupancy(c->lay, c->tile[rank], rank, &wgPerCu);
 	if (st != RPWA_OK) {
 		set_error("hipOccupancyMaxActiveBlocksPerMultiprocessor failed");
 		return st;
 	}
-	c->nWg = c->nCu * std::min(wgPerCu, 8);
-	const size_t need = (size_t)c->nWg * partial_elems(kMaxRank, c->lay.wmax);
+	c->nWg[rank] = c->nCu * std::min(wgPerCu, 8);
+	const size_t need = (size_t)c->nWg[rank] * partial_elems(kMaxRank, c->lay.wmax);
 	if (need > c->partialCap) {
 		hipFree(c->d_partials);
 		c->d_partials = nullptr;
@@ -138,11 +144,16 @@ int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
 	L.secOff[0] = 0;
 	L.secOff[1] = pad0;
 	L.rowLen = pad0 + pad1;
-	if (!choose_tile(L, c->tile)) {
+	TileShape ts;
+	if (!choose_tile(L, ts)) {
 		set_error("rpwa_lik: " + std::to_string(nW[0]) + "+" + std::to_string(nW[1]) +
 		          " waves exceed the register tiles (at most " + std::to_string(8 * 64 * L.perSlot) +
 		          " waves in total)");
 		return RPWA_ERR_UNSUPPORTED;
+	}
+	for (int r = 0; r <= kMaxRank; ++r) {
+		c->tile[r] = ts;
+		c->tileFixed[r] = false;
 	}
 	return RPWA_OK;
 }
@@ -155,7 +166,10 @@ int alloc_eval_buffers(rpwa_lik_ctx* c)
 	RPWA_HIP_CHECK(hipMalloc(&c->d_out, nOut * sizeof(double)));
 	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
 	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocDefault));
-	return configure_grid(c);
+	for (int r = 1; r <= kMaxRank; ++r)
+		if (int st = configure_grid(c, r))
+			return st;
+	return RPWA_OK;
 }
 
 // load from FP64 amplitudes already on the device
@@ -186,6 +200,72 @@ int load_from_device(rpwa_lik_ctx* c, const double* d_src, int srcLayout, uint64
 	return RPWA_OK;
 }
 
+// Measured tile choice for one rank: every feasible lane count
+// runs the event kernel a few times on the context's amplitudes with fixed
+// production amplitudes; the fastest (median of the timed launches) is kept.
+int autotune_rank(rpwa_lik_ctx* c, int rank)
+{
+	const uint32_t wmax = c->lay.wmax;
+	const size_t nProd = (size_t)rank * 2 * wmax;
+	std::vector<double2> prod(nProd);
+	for (size_t i = 0; i < nProd; ++i)
+		prod[i] = make_double2(0.5 + 0.01 * (double)(i % 7), 0.25 - 0.01 * (double)(i % 5));
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, prod.data(), nProd * sizeof(double2), hipMemcpyHostToDevice,
+	                              c->stream));
+	const uint32_t nElem = partial_elems(rank, wmax);
+	EventArgs a;
+	a.amps = c->d_amps;
+	a.nEvents = c->lay.nEvents;
+	a.rowLen = c->lay.rowLen;
+	a.secOff0 = c->lay.secOff[0];
+	a.secOff1 = c->lay.secOff[1];
+	a.nW0 = c->lay.nW[0];
+	a.nW1 = c->lay.nW[1];
+	a.wmax = wmax;
+	a.rank = rank;
+	a.prod = reinterpret_cast<const double2*>(c->d_prod);
+	a.flat2 = 0.01;
+	a.partials = c->d_partials;
+	a.nElem = nElem;
+	const TileShape keep = c->tile[rank];
+	TileShape best = keep;
+	double bestT = 1e300;
+	const int reps = 5;
+	for (int lanes : {8, 16, 32, 64}) {
+			TileShape ts;
+			if (!make_tile(c->lay, lanes, ts))
+				continue;
+			c->tile[rank] = ts;
+			if (int st = configure_grid(c, rank))
+				return st;
+			a.lanes0 = ts.lanes0;
+			a.lanes1 = ts.lanes1;
+			std::vector<float> ms;
+			for (int it = 0; it < reps + 1; ++it) {
+				RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+				RPWA_HIP_CHECK(launch_event_kernel(c->lay, ts, a, true, c->nWg[rank], c->stream));
+				RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+				RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
+				float t = 0.f;
+				RPWA_HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+				if (it > 0)
+					ms.push_back(t);
+			}
+			std::sort(ms.begin(), ms.end());
+			const double med = ms[ms.size() / 2];
+			if (std::getenv("RPWA_VERBOSE"))
+				std::fprintf(stderr, "[rpwa autotune] precision %d rank %d lanes %d slots %d (%d+%d lanes) "
+				             "workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots, ts.lanes0,
+				             ts.lanes1, c->nWg[rank], med);
+			if (med < bestT) {
+				bestT = med;
+				best = ts;
+			}
+		}
+	c->tile[rank] = best;
+	return configure_grid(c, rank);
+}
+
 // One evaluation: flags select what is copied back.
 int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
              double* lnl, double* deriv, double* derivFlat, const char* fn)
@@ -206,6 +286,12 @@ int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, doub
 		return RPWA_ERR_INVALID;
 	}
 	RPWA_HIP_CHECK(hipSetDevice(c->device));
+	if (!c->tileFixed[rank]) {
+		if (c->autotune)
+			if (int st = autotune_rank(c, (int)rank))
+				return st;
+		c->tileFixed[rank] = true;
+	}
 	std::memcpy(c->h_prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2));
 	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, (size_t)nmbProdAmps * sizeof(double2),
 	                              hipMemcpyHostToDevice, c->stream));
@@ -220,16 +306,16 @@ int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, doub
 	a.nW1 = c->lay.nW[1];
 	a.wmax = wmax;
 	a.rank = (int)rank;
-	a.slots = c->tile.slots;
-	a.slot1 = c->tile.slot1;
+	a.lanes0 = c->tile[rank].lanes0;
+	a.lanes1 = c->tile[rank].lanes1;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
 	a.flat2 = flat * flat;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
 	RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-	RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile, a, true, c->nWg, c->stream));
+	RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[rank], a, true, c->nWg[rank], c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
-	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg, nElem, flat, c->d_out, c->stream));
+	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, flat, c->d_out, c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
 	const size_t nRes = deriv ? (nElem - 1) : 2;
 	RPWA_HIP_CHECK(hipMemcpyAsync(c->h_out, c->d_out, nRes * sizeof(double), hipMemcpyDeviceToHost,
@@ -311,6 +397,8 @@ extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
 		e = hipEventCreate(&c->evMid);
 	if (e == hipSuccess)
 		e = hipDeviceGetAttribute(&c->nCu, hipDeviceAttributeMultiprocessorCount, device);
+	if (const char* env = std::getenv("RPWA_AUTOTUNE"))
+		c->autotune = std::string(env) != "0";
 	if (e != hipSuccess) {
 		set_error(std::string("rpwa_lik_create: ") + hipGetErrorString(e));
 		if (c->stream)
@@ -509,9 +597,13 @@ extern "C" int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9])
 	info[1] = ctx->lay.nW[0];
 	info[2] = ctx->lay.nW[1];
 	info[3] = (uint64_t)ctx->precision;
-	info[4] = (uint64_t)ctx->tile.lanes;
-	info[5] = (uint64_t)ctx->tile.slots;
-	info[6] = (uint64_t)ctx->nWg;
+	int r = 1;
+	for (int k = kMaxRank; k >= 1; --k)
+		if (ctx->tileFixed[k])
+			r = k;  // lowest rank evaluated so far (1 if none)
+	info[4] = (uint64_t)ctx->tile[r].lanes;
+	info[5] = (uint64_t)ctx->tile[r].slots;
+	info[6] = (uint64_t)ctx->nWg[r];
 	info[7] = 256;
 	info[8] = ctx->lay.bytes();
 	return RPWA_OK;
@@ -526,25 +618,36 @@ extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
 		set_error("rpwa_lik_set_tile: decay amplitudes not loaded");
 		return RPWA_ERR_STATE;
 	}
-	if (lanesPerEvent != 8 && lanesPerEvent != 16 && lanesPerEvent != 32 && lanesPerEvent != 64) {
+	const uint32_t lanes = lanesPerEvent;
+	if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
 		set_error("rpwa_lik_set_tile: lanes per event must be 8, 16, 32 or 64");
 		return RPWA_ERR_INVALID;
 	}
 	TileShape ts;
-	if (!make_tile(ctx->lay, (int)lanesPerEvent, ts)) {
-		set_error("rpwa_lik_set_tile: the amplitude row does not fit " + std::to_string(lanesPerEvent) +
+	if (!make_tile(ctx->lay, (int)lanes, ts)) {
+		set_error("rpwa_lik_set_tile: the amplitude row does not fit " + std::to_string(lanes) +
 		          " lanes x 8 slots");
 		return RPWA_ERR_UNSUPPORTED;
 	}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
-	const TileShape old = ctx->tile;
-	ctx->tile = ts;
-	const int st = configure_grid(ctx);
-	if (st != RPWA_OK)
-		ctx->tile = old;
-	return st;
+	for (int r = 1; r <= kMaxRank; ++r) {
+		ctx->tile[r] = ts;
+		ctx->tileFixed[r] = true;
+		if (int st = configure_grid(ctx, r))
+			return st;
+	}
+	return RPWA_OK;
 	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_set_autotune"))
+		return st;
+	ctx->autotune = enable != 0;
+	return RPWA_OK;
 }
 
 extern "C" int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream)

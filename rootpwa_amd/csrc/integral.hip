@@ -27,8 +27,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIntThreads = 256;
 constexpr int kIntWaves = 4;
-constexpr int kChunk = 16;       // events per staged chunk (4 MFMA k-steps)
+constexpr int kChunk = 32;         // events per staged chunk (8 MFMA k-steps)
 constexpr int kMaxTilesPerWave = 24;
+constexpr int kMaxLoadsPerThread = 16;  // double2 loads per thread and chunk (D <= 256)
 
 struct IntArgs {
 	const double* x;       // [M][D] row-major (interleaved complex rows)
@@ -58,79 +59,125 @@ __device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, ui
 	J = i + rem;
 }
 
+// NT tiles per wave: wave w owns the contiguous block [w*NT, (w+1)*NT) of the
+// pass; slots past the end of the pass compute a dummy tile that is discarded,
+// so the MFMA loop has no data-dependent exits and the accumulators stay in
+// registers.
 template <int NT>
 __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 {
-	extern __shared__ double xs[];   // [kChunk][ldsStride] + weights[kChunk]
-	double* wts = xs + (size_t)kChunk * a.ldsStride;
+	extern __shared__ double lds[];   // 2 x ([kChunk][ldsStride] + weights[kChunk])
+	const uint32_t bufDoubles = kChunk * a.ldsStride + kChunk;
 	const int tid = threadIdx.x;
 	const int lane = tid & 63;
 	const int wid = tid >> 6;
 	const int li = lane & 15;   // row/column within the tile
 	const int lk = lane >> 4;   // k index within a 4-event step
 
-	// tiles of this wave: t = wid, wid + 4, ...
-	d4 acc[NT];
 	uint32_t tI[NT], tJ[NT];
-	int nMine = 0;
+	bool keep[NT];
+	d4 acc[NT];
+#pragma unroll
 	for (int q = 0; q < NT; ++q) {
-		const uint32_t t = a.tileBase + (uint32_t)wid + (uint32_t)q * kIntWaves;
+		const uint32_t t = a.tileBase + (uint32_t)(wid * NT + q);
 		uint32_t I = 0, J = 0;
-		if (t < a.tileEnd) {
+		keep[q] = t < a.tileEnd;
+		if (keep[q])
 			tile_of(t, a.nT, I, J);
-			nMine = q + 1;
-		}
-		tI[q] = I;
-		tJ[q] = J;
+		tI[q] = I * 16 + li;
+		tJ[q] = J * 16 + li;
 		acc[q] = d4{0., 0., 0., 0.};
 	}
-	double wacc = 0.;
 
+	// staging geometry: thread loads double2 number tid + 256 l of the chunk
+	const uint32_t chunkD2 = kChunk * a.D / 2;
+	uint32_t ldsOff[kMaxLoadsPerThread];
+#pragma unroll
+	for (int l = 0; l < kMaxLoadsPerThread; ++l) {
+		const uint32_t idx = 2 * ((uint32_t)tid + (uint32_t)l * kIntThreads);
+		ldsOff[l] = (idx / a.D) * a.ldsStride + idx % a.D;
+	}
+	// zero both buffers once: padding columns (D..Dp) stay zero
+	for (uint32_t i = tid; i < 2 * bufDoubles; i += kIntThreads)
+		lds[i] = 0.;
+	__syncthreads();
+
+	double wacc = 0.;
 	const uint64_t nChunks = (a.M + kChunk - 1) / kChunk;
-	for (uint64_t c = blockIdx.x; c < nChunks; c += gridDim.x) {
+	double2 stage[kMaxLoadsPerThread];
+	double wstage = 0.;
+	auto load_chunk = [&](uint64_t c) {
 		const uint64_t e0 = c * kChunk;
-		__syncthreads();
-		// stage the chunk: rows e0..e0+15, columns 0..Dp-1 (zero padded)
-		for (uint32_t idx = tid; idx < (uint32_t)kChunk * a.Dp; idx += kIntThreads) {
-			const uint32_t r = idx / a.Dp;
-			const uint32_t col = idx % a.Dp;
-			const uint64_t e = e0 + r;
-			double v = 0.;
-			if (e < a.M && col < a.D)
-				v = a.x[e * a.D + col];
-			xs[(size_t)r * a.ldsStride + col] = v;
+		const uint64_t nEv = a.M - e0 < (uint64_t)kChunk ? a.M - e0 : (uint64_t)kChunk;
+		const uint32_t lim = (uint32_t)nEv * a.D / 2;
+		const double2* src = reinterpret_cast<const double2*>(a.x + e0 * a.D);
+#pragma unroll
+		for (int l = 0; l < kMaxLoadsPerThread; ++l) {
+			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
+			stage[l] = (i2 < lim) ? src[i2] : make_double2(0., 0.);
 		}
 		if (tid < kChunk) {
 			const uint64_t e = e0 + tid;
-			double wt = 0.;
-			if (e < a.M)
-				wt = a.w ? 1. / a.w[e] : 1.;
-			wts[tid] = wt;
-			wacc += wt;
+			wstage = e < a.M ? (a.w ? 1. / a.w[e] : 1.) : 0.;
 		}
-		__syncthreads();
+	};
+	auto store_chunk = [&](int buf) {
+		double* xs = lds + (size_t)buf * bufDoubles;
+#pragma unroll
+		for (int l = 0; l < kMaxLoadsPerThread; ++l) {
+			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
+			if (i2 < chunkD2) {
+				xs[ldsOff[l]] = stage[l].x;
+				xs[ldsOff[l] + 1] = stage[l].y;
+			}
+		}
+		if (tid < kChunk) {
+			xs[kChunk * a.ldsStride + tid] = wstage;
+			wacc += wstage;
+		}
+	};
+
+	uint64_t c = blockIdx.x;
+	int buf = 0;
+	if (c < nChunks) {
+		load_chunk(c);
+		store_chunk(0);
+	}
+	__syncthreads();
+	for (; c < nChunks; c += gridDim.x) {
+		const uint64_t next = c + gridDim.x;
+		if (next < nChunks)
+			load_chunk(next);  // in flight while the MFMAs below run
+		const double* xs = lds + (size_t)buf * bufDoubles;
+		const double* wts = xs + kChunk * a.ldsStride;
+#pragma unroll
 		for (int ks = 0; ks < kChunk / 4; ++ks) {
 			const int row = ks * 4 + lk;
 			const double wt = wts[row];
 			const double* xr = xs + (size_t)row * a.ldsStride;
+#pragma unroll
 			for (int q = 0; q < NT; ++q) {
-				if (q >= nMine)
-					break;
-				const double av = xr[tI[q] * 16 + li] * wt;
-				const double bv = xr[tJ[q] * 16 + li];
+				const double av = xr[tI[q]] * wt;
+				const double bv = xr[tJ[q]];
 				acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
 			}
 		}
+		if (next < nChunks)
+			store_chunk(buf ^ 1);
+		__syncthreads();
+		buf ^= 1;
 	}
 
 	// v_mfma_f64_16x16x4_f64 result layout (checked on gfx950):
 	// D[i][j] sits in lane j + 16 * (i % 4), register i / 4
 	double* out = a.partials + (size_t)blockIdx.x * a.nTiles * 256;
+#pragma unroll
 	for (int q = 0; q < NT; ++q) {
-		if (q >= nMine)
-			break;
-		const uint32_t t = a.tileBase + (uint32_t)wid + (uint32_t)q * kIntWaves;
+		if (!keep[q])
+			continue;
+		const uint32_t t = a.tileBase + (uint32_t)(wid * NT + q);
 		double* o = out + (size_t)t * 256;
+#pragma unroll
 		for (int v = 0; v < 4; ++v) {
 			const int i = lk + 4 * v;
 			const int j = li;
@@ -194,7 +241,12 @@ static int integral_plan(int device, uint32_t nWaves, IntegralPlan& p)
 	p.nTiles = p.nT * (p.nT + 1) / 2;
 	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
 	p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
-	p.lds = ((size_t)kChunk * p.ldsStride + kChunk) * sizeof(double);
+	p.lds = 2 * ((size_t)kChunk * p.ldsStride + kChunk) * sizeof(double);
+	if ((size_t)kChunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
+		set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / kChunk) +
+		          " waves per amplitude row");
+		return RPWA_ERR_UNSUPPORTED;
+	}
 	int nCu = 0;
 	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
 	p.nWg = nCu > 0 ? nCu : 1;
@@ -206,8 +258,10 @@ static IntKernel pick_int_kernel(uint32_t nTiles)
 {
 	const uint32_t per = (nTiles + kIntWaves - 1) / kIntWaves;
 	if (per <= 1) return integral_kernel<1>;
+	if (per <= 2) return integral_kernel<2>;
 	if (per <= 4) return integral_kernel<4>;
-	if (per <= 8) return integral_kernel<8>;
+	if (per <= 6) return integral_kernel<6>;
+	if (per <= 9) return integral_kernel<9>;
 	if (per <= 12) return integral_kernel<12>;
 	if (per <= 17) return integral_kernel<17>;
 	return integral_kernel<24>;

@@ -42,12 +42,14 @@ struct AmpLayout {
 	size_t bytes() const { return (size_t)nEvents * rowLen * (precision == RPWA_PREC_FP64 ? 16 : 8); }
 };
 
-// Per-launch description of the event tiling.
+// Per-launch description of the event tiling: a group of `lanes` lanes owns
+// one event; lanes0 of them hold the refl -1 waves, lanes1 the refl +1 waves,
+// each lane `slots` slots of perSlot complex values.
 struct TileShape {
-	int lanes = 16;        // lanes cooperating on one event
-	int slots = 1;         // slots per lane (each perSlot complex values)
-	int kmax = 2;          // register capacity (template) >= slots
-	int slot1 = 0;         // first slot of sector 1
+	int lanes = 16;
+	int slots = 1;
+	int lanes0 = 0;
+	int lanes1 = 0;
 };
 
 struct EventArgs {
@@ -58,8 +60,7 @@ struct EventArgs {
 	uint32_t nW0, nW1;
 	uint32_t wmax;
 	int rank;
-	int slots;
-	int slot1;
+	int lanes0, lanes1;
 	const double2* prod;   // device [rank][2][wmax] complex
 	double flat2;
 	double* partials;      // [nWg][nElem]
@@ -73,7 +74,7 @@ inline uint32_t partial_elems(int rank, uint32_t wmax) { return 3u + 2u * (uint3
 // launchers (lik_kernels.hip)
 hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
                                bool grad, int nWg, hipStream_t s);
-int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, bool grad, int* wgPerCu);
+int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s);
 hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int srcLayout,
@@ -88,10 +89,14 @@ struct rpwa_lik_ctx {
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, evMid = nullptr, ev1 = nullptr;
 	rpwa_amd::AmpLayout lay;
-	rpwa_amd::TileShape tile;
+	// tile and persistent grid per rank (index = rank); fixed before the first
+	// evaluation of that rank, so every result of a rank comes from one tile
+	rpwa_amd::TileShape tile[rpwa_amd::kMaxRank + 1];
+	int nWg[rpwa_amd::kMaxRank + 1] = {0, 0, 0, 0, 0};
+	bool tileFixed[rpwa_amd::kMaxRank + 1] = {false, false, false, false, false};
+	bool autotune = true;
 	bool loaded = false;
 	int nCu = 0;
-	int nWg = 0;
 	void* d_amps = nullptr;
 	double* d_prod = nullptr;      // [kMaxRank][2][wmax] complex
 	double* d_partials = nullptr;  // [nWg][partial_elems]
