@@ -98,6 +98,17 @@ int rpwa_lik_fdf(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps
                  double prodAmpFlat, uint32_t rank, double* logLikelihood,
                  double* derivatives, double* derivativeFlat);
 
+/* Asynchronous form of rpwa_lik_fdf: rpwa_lik_submit starts the evaluation on
+ * the context's stream and returns at once (the production amplitudes are
+ * copied before it returns); rpwa_lik_collect waits for it and writes the
+ * results (any output may be NULL).  At most one evaluation per context may be
+ * pending; the results are bitwise those of rpwa_lik_fdf.  Lets the host work
+ * on the normalisation term while the GPU streams the amplitudes. */
+int rpwa_lik_submit(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                    double prodAmpFlat, uint32_t rank);
+int rpwa_lik_collect(rpwa_lik_ctx* ctx, double* logLikelihood, double* derivatives,
+                     double* derivativeFlat);
+
 /* Event part of the Hessian (pwaLikelihood.cc:697-753): for a, b over the
  * [rank][2][nmbWavesMax] production amplitudes, hessian[a][b][3] holds the
  * real/real, real/imag and imag/imag event sums; flatTerms[a] (complex) the

@@ -200,9 +200,9 @@ private:
 
 	// host parts of the evaluation (reference arithmetic)
 	double normTerm(const std::vector<std::complex<double> >& prodAmps, const double prodAmpFlat,
-	                std::vector<std::complex<double> >* derivatives, double* derivativeFlat) const;
+	                std::vector<std::complex<double> >* normDeriv, double* normDerivFlat) const;
 	double priorTerm(const std::vector<std::complex<double> >& prodAmps,
-	                 std::vector<std::complex<double> >* derivatives) const;
+	                 std::vector<std::complex<double> >* priorDeriv) const;
 	// event sums on the GPU; deriv may be null (value only)
 	void eventSums(const std::vector<std::complex<double> >& prodAmps, const double prodAmpFlat, double* lnl,
 	               std::vector<std::complex<double> >* deriv, double* derivFlat) const;

@@ -49,6 +49,8 @@ PROTOTYPES = {
     "rpwa_lik_log_likelihood_deriv": (_i, [_vp, _dp, _u32, _d, _u32, _dp, _dp]),
     "rpwa_lik_fdf": (_i, [_vp, _dp, _u32, _d, _u32, _dp, _dp, _dp]),
     "rpwa_lik_hessian": (_i, [_vp, _dp, _u32, _d, _u32, _dp, _dp, _dp]),
+    "rpwa_lik_submit": (_i, [_vp, _dp, _u32, _d, _u32]),
+    "rpwa_lik_collect": (_i, [_vp, _dp, _dp, _dp]),
     "rpwa_lik_kernel_time": (_i, [_vp, _dp]),
     "rpwa_lik_reset_kernel_time": (_i, [_vp]),
     "rpwa_lik_kernel_times": (_i, [_vp, _dp, _dp, _u64p]),
@@ -188,6 +190,19 @@ class LikelihoodContext:
         df = ctypes.c_double()
         f = ctypes.c_double()
         call("rpwa_lik_fdf", self.h, _cd(p), p.size, float(flat), rank, ctypes.byref(f), _cd(d), ctypes.byref(df))
+        return f.value, d, df.value
+
+    def submit(self, prod, flat):
+        p, rank = self._prod(prod)
+        self._pending = p
+        call("rpwa_lik_submit", self.h, _cd(p), p.size, float(flat), rank)
+
+    def collect(self):
+        p = self._pending
+        d = np.zeros_like(p)
+        df = ctypes.c_double()
+        f = ctypes.c_double()
+        call("rpwa_lik_collect", self.h, ctypes.byref(f), _cd(d), ctypes.byref(df))
         return f.value, d, df.value
 
     def hessian(self, prod, flat):

@@ -12,6 +12,14 @@
 
 namespace rpwa {
 
+namespace {
+// complex / real, component-wise (as GCC divides std::complex<double> by a double)
+inline std::complex<double> cdiv(const std::complex<double>& z, const double d)
+{
+	return std::complex<double>(z.real() / d, z.imag() / d);
+}
+}  // namespace
+
 ampIntegralMatrix::ampIntegralMatrix() : _nmbEvents(0) {}
 
 // ampIntegralMatrix::setWaveNames (ampIntegralMatrix.cc:255-277)
@@ -64,7 +72,7 @@ std::complex<double> ampIntegralMatrix::element(const unsigned int waveIndexI, c
 	const unsigned int W = nmbWaves();
 	if (waveIndexI >= W or waveIndexJ >= W)
 		throw std::out_of_range("ampIntegralMatrix::element: wave index out of range");
-	return _integrals[(size_t)waveIndexI * W + waveIndexJ] / ((double)_nmbEvents);
+	return cdiv(_integrals[(size_t)waveIndexI * W + waveIndexJ], (double)_nmbEvents);
 }
 
 bool ampIntegralMatrix::integrate(const std::complex<double>* amps, const unsigned long nmbEvents,
@@ -167,7 +175,7 @@ ampIntegralMatrix& ampIntegralMatrix::operator*=(const double factor)
 ampIntegralMatrix& ampIntegralMatrix::operator/=(const double factor)
 {
 	for (std::complex<double>& v : _integrals)
-		v /= factor;
+		v = cdiv(v, factor);
 	return *this;
 }
 

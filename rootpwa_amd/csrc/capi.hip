@@ -266,12 +266,18 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	return configure_grid(c, rank);
 }
 
-// One evaluation: flags select what is copied back.
-int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
-             double* lnl, double* deriv, double* derivFlat, const char* fn)
+// Start one evaluation on the context's stream; results are collected by
+// collect().  The whole result row (value, flat derivative, derivatives) is
+// always copied back: a few kB.
+int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
+           const char* fn)
 {
 	if (!c->loaded) {
 		set_error(std::string(fn) + ": decay amplitudes not loaded");
+		return RPWA_ERR_STATE;
+	}
+	if (c->pending) {
+		set_error(std::string(fn) + ": an evaluation is still pending (rpwa_lik_collect first)");
 		return RPWA_ERR_STATE;
 	}
 	if (!prodAmps || rank == 0 || rank > (uint32_t)kMaxRank) {
@@ -317,9 +323,22 @@ int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, doub
 	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, flat, c->d_out, c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-	const size_t nRes = deriv ? (nElem - 1) : 2;
-	RPWA_HIP_CHECK(hipMemcpyAsync(c->h_out, c->d_out, nRes * sizeof(double), hipMemcpyDeviceToHost,
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->h_out, c->d_out, (nElem - 1) * sizeof(double), hipMemcpyDeviceToHost,
 	                              c->stream));
+	c->pending = true;
+	c->pendingRank = rank;
+	c->pendingProd = nmbProdAmps;
+	return RPWA_OK;
+}
+
+int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, const char* fn)
+{
+	if (!c->pending) {
+		set_error(std::string(fn) + ": no evaluation pending");
+		return RPWA_ERR_STATE;
+	}
+	c->pending = false;
+	RPWA_HIP_CHECK(hipSetDevice(c->device));
 	RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
 	float ms = 0.f, ms1 = 0.f;
 	if (hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
@@ -334,8 +353,16 @@ int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, doub
 	if (derivFlat)
 		*derivFlat = c->h_out[1];
 	if (deriv)
-		std::memcpy(deriv, c->h_out + 2, (size_t)nmbProdAmps * sizeof(double2));
+		std::memcpy(deriv, c->h_out + 2, (size_t)c->pendingProd * sizeof(double2));
 	return RPWA_OK;
+}
+
+int evaluate(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
+             double* lnl, double* deriv, double* derivFlat, const char* fn)
+{
+	if (int st = submit(c, prodAmps, nmbProdAmps, flat, rank, fn))
+		return st;
+	return collect(c, lnl, deriv, derivFlat, fn);
 }
 
 }  // namespace
@@ -541,6 +568,28 @@ extern "C" int rpwa_lik_fdf(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t 
 	RPWA_GUARD_BEGIN
 	return evaluate(ctx, prodAmps, nmbProdAmps, prodAmpFlat, rank, logLikelihood, derivatives,
 	                derivativeFlat, "rpwa_lik_fdf");
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_submit(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                               double prodAmpFlat, uint32_t rank)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_submit"))
+		return st;
+	RPWA_GUARD_BEGIN
+	return submit(ctx, prodAmps, nmbProdAmps, prodAmpFlat, rank, "rpwa_lik_submit");
+	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_collect(rpwa_lik_ctx* ctx, double* logLikelihood, double* derivatives,
+                                double* derivativeFlat)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_collect"))
+		return st;
+	RPWA_GUARD_BEGIN
+	return collect(ctx, logLikelihood, derivatives, derivativeFlat, "rpwa_lik_collect");
 	RPWA_GUARD_END
 }
 

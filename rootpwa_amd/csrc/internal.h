@@ -104,6 +104,10 @@ struct rpwa_lik_ctx {
 	double* h_prod = nullptr;      // pinned staging
 	double* h_out = nullptr;       // pinned staging
 	size_t partialCap = 0;
+	// pending asynchronous evaluation (rpwa_lik_submit / rpwa_lik_collect)
+	bool pending = false;
+	uint32_t pendingRank = 0;
+	uint32_t pendingProd = 0;
 	double kernelTime = 0;
 	double eventTime = 0;
 	double reduceTime = 0;

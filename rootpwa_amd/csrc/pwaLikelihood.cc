@@ -58,6 +58,13 @@ double cauchyFunctionSecondDerivative(const double x, const double gamma)
 	return gamma2 / ((gamma2 + x2) * (gamma2 + x2)) * (-2. + 8. * x2 / (gamma2 + x2));
 }
 
+// complex / real, component-wise: what GCC emits for std::complex<double>
+// divided by a double in the reference build (clang would divide by d + 0i)
+inline std::complex<double> cdiv(const std::complex<double>& z, const double d)
+{
+	return std::complex<double>(z.real() / d, z.imag() / d);
+}
+
 double seconds_since(const std::chrono::steady_clock::time_point& t0)
 {
 	return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -367,7 +374,7 @@ bool pwaLikelihood<complexT>::addAmplitude(const std::string& waveName, const st
 				return false;
 			}
 			if (normInt != 0.)
-				a /= std::sqrt(normInt.real());
+				a = cdiv(a, std::sqrt(normInt.real()));
 		}
 		if (a != 0.)
 			fixPar = false;
@@ -452,9 +459,10 @@ bool pwaLikelihood<complexT>::finishInit()
 						double norm_j = std::sqrt(_normMatrix[matIndex(jRefl, jWave, jRefl, jWave)].real());
 						if (norm_j == 0)
 							norm_j = 1;
+						const size_t ij = matIndex(iRefl, iWave, jRefl, jWave);
 						if (iRefl != jRefl or iWave != jWave)
-							_normMatrix[matIndex(iRefl, iWave, jRefl, jWave)] /= norm_i * norm_j;
-						_accMatrix[matIndex(iRefl, iWave, jRefl, jWave)] /= norm_i * norm_j;
+							_normMatrix[ij] = cdiv(_normMatrix[ij], norm_i * norm_j);
+						_accMatrix[ij] = cdiv(_accMatrix[ij], norm_i * norm_j);
 					}
 			}
 		for (unsigned int r = 0; r < 2; ++r)
@@ -545,15 +553,20 @@ void pwaLikelihood<complexT>::copyToParArray(const std::vector<std::complex<doub
 // ---------------------------------------------------------------------------
 // host terms
 
-// normalisation term and its derivative (FdF :255-279, DoEval :384-399, Gradient :588-606)
+// normalisation term and its derivative (FdF :255-279, DoEval :384-399, Gradient :588-606).
+// The derivative contributions (sum_j T_j conj(I_ij)) * 2nu and flat * 2nu * eta
+// are written to normDeriv / normDerivFlat, to be added to the event sums in the
+// reference's order once those arrive from the GPU.
 template <typename complexT>
 double pwaLikelihood<complexT>::normTerm(const std::vector<std::complex<double> >& prodAmps, const double prodAmpFlat,
-                                         std::vector<std::complex<double> >* derivatives,
-                                         double* derivativeFlat) const
+                                         std::vector<std::complex<double> >* normDeriv,
+                                         double* normDerivFlat) const
 {
 	kahanSum normFactorAcc;
 	const double nmbEvt = _useNormalizedAmps ? 1 : (double)_nmbEvents;
 	const double twiceNmbEvt = 2 * nmbEvt;
+	if (normDeriv)
+		normDeriv->assign(prodAmps.size(), std::complex<double>(0, 0));
 	for (unsigned int iRank = 0; iRank < _rank; ++iRank)
 		for (unsigned int iRefl = 0; iRefl < 2; ++iRefl)
 			for (unsigned int iWave = 0; iWave < _nmbWavesRefl[iRefl]; ++iWave) {
@@ -563,27 +576,32 @@ double pwaLikelihood<complexT>::normTerm(const std::vector<std::complex<double> 
 					const std::complex<double> I = _accMatrix[matIndex(iRefl, iWave, iRefl, jWave)];
 					const std::complex<double> Tj = prodAmps[prodIndex(iRank, iRefl, jWave)];
 					normFactorAcc(std::real((Ti * std::conj(Tj)) * I));
-					if (derivatives)
+					if (normDeriv)
 						normFactorDerivAcc(Tj * std::conj(I));
 				}
-				if (derivatives)
-					(*derivatives)[prodIndex(iRank, iRefl, iWave)] += normFactorDerivAcc.sum() * twiceNmbEvt;
+				if (normDeriv)
+					(*normDeriv)[prodIndex(iRank, iRefl, iWave)] = normFactorDerivAcc.sum() * twiceNmbEvt;
 			}
 	const double prodAmpFlat2 = prodAmpFlat * prodAmpFlat;
 	normFactorAcc(prodAmpFlat2 * _totAcc);
-	if (derivativeFlat)
-		*derivativeFlat += prodAmpFlat * twiceNmbEvt * _totAcc;
+	if (normDerivFlat)
+		*normDerivFlat = prodAmpFlat * twiceNmbEvt * _totAcc;
 	return normFactorAcc.sum();
 }
 
-// half-Cauchy prior (FdF :281-297, DoEval :401-417, Gradient :608-625)
+// half-Cauchy prior (FdF :281-297, DoEval :401-417, Gradient :608-625); the
+// derivative contributions go to priorDeriv (subtracted later, as the reference does)
 template <typename complexT>
 double pwaLikelihood<complexT>::priorTerm(const std::vector<std::complex<double> >& prodAmps,
-                                          std::vector<std::complex<double> >* derivatives) const
+                                          std::vector<std::complex<double> >* priorDeriv) const
 {
 	double priorValue = 0.;
+	if (priorDeriv)
+		priorDeriv->clear();
 	if (_priorType != HALF_CAUCHY)
 		return priorValue;
+	if (priorDeriv)
+		priorDeriv->assign(prodAmps.size(), std::complex<double>(0, 0));
 	for (unsigned int iRank = 0; iRank < _rank; ++iRank)
 		for (unsigned int iRefl = 0; iRefl < 2; ++iRefl)
 			for (unsigned int iWave = 0; iWave < _nmbWavesRefl[iRefl]; ++iWave) {
@@ -591,13 +609,26 @@ double pwaLikelihood<complexT>::priorTerm(const std::vector<std::complex<double>
 				const double r = std::abs(t);
 				const double cauchyFunctionValue = cauchyFunction(r, _cauchyWidth);
 				priorValue -= std::log(cauchyFunctionValue);
-				if (derivatives) {
+				if (priorDeriv) {
 					const double factor = (1. / (r * cauchyFunctionValue)) * cauchyFunctionDerivative(r, _cauchyWidth);
-					(*derivatives)[prodIndex(iRank, iRefl, iWave)] -=
+					(*priorDeriv)[prodIndex(iRank, iRefl, iWave)] =
 						std::complex<double>(factor * t.real(), factor * t.imag());
 				}
 			}
 	return priorValue;
+}
+
+// derivatives = event sums + norm term - prior, in the reference's order
+template <typename complexT>
+static void combine_derivatives(std::vector<std::complex<double> >& derivatives,
+                                const std::vector<std::complex<double> >& normDeriv,
+                                const std::vector<std::complex<double> >& priorDeriv)
+{
+	for (size_t i = 0; i < derivatives.size(); ++i)
+		derivatives[i] += normDeriv[i];
+	if (not priorDeriv.empty())
+		for (size_t i = 0; i < derivatives.size(); ++i)
+			derivatives[i] -= priorDeriv[i];
 }
 
 template <typename complexT>
@@ -610,17 +641,14 @@ void pwaLikelihood<complexT>::eventSums(const std::vector<std::complex<double> >
 	if (deriv) {
 		deriv->assign(prodAmps.size(), std::complex<double>(0, 0));
 		double* d = reinterpret_cast<double*>(deriv->data());
-		if (lnl)
-			check_status(rpwa_lik_fdf(_ctx.get(), p, n, prodAmpFlat, _rank, lnl, d, derivFlat), "rpwa_lik_fdf");
-		else
-			check_status(rpwa_lik_log_likelihood_deriv(_ctx.get(), p, n, prodAmpFlat, _rank, d, derivFlat),
-			             "rpwa_lik_log_likelihood_deriv");
+		check_status(rpwa_lik_fdf(_ctx.get(), p, n, prodAmpFlat, _rank, lnl, d, derivFlat), "rpwa_lik_fdf");
 	} else {
 		check_status(rpwa_lik_log_likelihood(_ctx.get(), p, n, prodAmpFlat, _rank, lnl), "rpwa_lik_log_likelihood");
 	}
 }
 
-// FdF (pwaLikelihood.cc:172-315)
+// FdF (pwaLikelihood.cc:172-315): the event sums run on the GPU while the host
+// computes the normalisation term and the prior
 template <typename complexT>
 void pwaLikelihood<complexT>::fdfImpl(const double* par, double& funcVal, double* gradient,
                                       const functionCallEnum who) const
@@ -629,20 +657,29 @@ void pwaLikelihood<complexT>::fdfImpl(const double* par, double& funcVal, double
 	std::vector<std::complex<double> > prodAmps;
 	double prodAmpFlat;
 	copyFromParArray(par, prodAmps, prodAmpFlat);
-	std::vector<std::complex<double> > derivatives;
-	double derivativeFlat = 0, logLikelihood = 0;
-	eventSums(prodAmps, prodAmpFlat, &logLikelihood, &derivatives, &derivativeFlat);
-	const double tFunc = seconds_since(t0);
+	check_status(rpwa_lik_submit(_ctx.get(), reinterpret_cast<const double*>(prodAmps.data()),
+	                             (uint32_t)prodAmps.size(), prodAmpFlat, _rank),
+	             "rpwa_lik_submit");
 	const auto t1 = std::chrono::steady_clock::now();
 	const double nmbEvt = _useNormalizedAmps ? 1 : (double)_nmbEvents;
-	const double normFactor = normTerm(prodAmps, prodAmpFlat, &derivatives, &derivativeFlat);
+	std::vector<std::complex<double> > normDeriv, priorDeriv;
+	double normDerivFlat = 0;
+	const double normFactor = normTerm(prodAmps, prodAmpFlat, &normDeriv, &normDerivFlat);
 	const double tNorm = seconds_since(t1);
-	const double priorValue = priorTerm(prodAmps, &derivatives);
+	const double priorValue = priorTerm(prodAmps, &priorDeriv);
+	std::vector<std::complex<double> > derivatives(prodAmps.size());
+	double derivativeFlat = 0, logLikelihood = 0;
+	check_status(rpwa_lik_collect(_ctx.get(), &logLikelihood, reinterpret_cast<double*>(derivatives.data()),
+	                              &derivativeFlat),
+	             "rpwa_lik_collect");
+	combine_derivatives<complexT>(derivatives, normDeriv, priorDeriv);
+	derivativeFlat += normDerivFlat;
 	copyToParArray(derivatives, derivativeFlat, gradient);
 	funcVal = logLikelihood + nmbEvt * normFactor + priorValue;
-	_funcCallInfo[who].funcTime += tFunc;
+	const double tTot = seconds_since(t0);
+	_funcCallInfo[who].funcTime += tTot - tNorm;
 	_funcCallInfo[who].normTime += tNorm;
-	_funcCallInfo[who].totalTime += seconds_since(t0);
+	_funcCallInfo[who].totalTime += tTot;
 	if (_debug)
 		std::cout << "raw log likelihood = " << std::setprecision(17) << logLikelihood
 		          << ", normalization = " << normFactor << ", prior = " << priorValue
@@ -719,16 +756,25 @@ void pwaLikelihood<complexT>::Gradient(const double* par, double* gradient) cons
 		std::vector<std::complex<double> > prodAmps;
 		double prodAmpFlat;
 		copyFromParArray(par, prodAmps, prodAmpFlat);
-		std::vector<std::complex<double> > derivatives;
-		double derivativeFlat = 0;
-		eventSums(prodAmps, prodAmpFlat, nullptr, &derivatives, &derivativeFlat);
-		_funcCallInfo[GRADIENT].funcTime += seconds_since(t0);
+		check_status(rpwa_lik_submit(_ctx.get(), reinterpret_cast<const double*>(prodAmps.data()),
+		                             (uint32_t)prodAmps.size(), prodAmpFlat, _rank),
+		             "rpwa_lik_submit");
 		const auto t1 = std::chrono::steady_clock::now();
-		normTerm(prodAmps, prodAmpFlat, &derivatives, &derivativeFlat);
+		std::vector<std::complex<double> > normDeriv, priorDeriv;
+		double normDerivFlat = 0;
+		normTerm(prodAmps, prodAmpFlat, &normDeriv, &normDerivFlat);
 		_funcCallInfo[GRADIENT].normTime += seconds_since(t1);
-		priorTerm(prodAmps, &derivatives);
+		priorTerm(prodAmps, &priorDeriv);
+		std::vector<std::complex<double> > derivatives(prodAmps.size());
+		double derivativeFlat = 0;
+		check_status(rpwa_lik_collect(_ctx.get(), nullptr, reinterpret_cast<double*>(derivatives.data()),
+		                              &derivativeFlat),
+		             "rpwa_lik_collect");
+		combine_derivatives<complexT>(derivatives, normDeriv, priorDeriv);
+		derivativeFlat += normDerivFlat;
 		copyToParArray(derivatives, derivativeFlat, gradient);
 		_funcCallInfo[GRADIENT].totalTime += seconds_since(t0);
+		_funcCallInfo[GRADIENT].funcTime += seconds_since(t0);
 	}
 	std::copy(gradient, gradient + _nmbPars, _derivCache.begin());
 }

@@ -1,0 +1,151 @@
+"""The C ABI without a GPU: the library loads, exports every declared entry
+point, and its host-side logic (parameter mapping, sign convention, integral
+matrix bookkeeping, error paths) matches the oracle.  No kernel runs here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("rootpwa_amd.h", "rootpwa_amd_pwalik.h")]
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(rpwa_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from rootpwa_amd import capi
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = [n for h in HEADERS for n in declared_functions(h)]
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_python_bindings_cover_the_headers():
+    from rootpwa_amd import capi, likelihood
+    names = [n for h in HEADERS for n in declared_functions(h)]
+    bound = set(capi.PROTOTYPES) | set(likelihood.PROTOTYPES)
+    assert [n for n in names if n not in bound] == []
+
+
+def test_errors_without_gpu():
+    from rootpwa_amd import capi
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(capi.RpwaError) as e:
+        capi.LikelihoodContext()
+    assert e.value.code == capi.RPWA_ERR_INVALID and "no GPU" in str(e.value)
+    # null context is an error, never a crash
+    assert capi.lib().rpwa_lik_release(None) == capi.RPWA_ERR_INVALID
+    assert b"null context" in capi.lib().rpwa_last_error()
+
+
+def wave_list(nW):
+    return [("neg%02d" % i, -1, 0.0) for i in range(nW[0])] + [("pos%02d" % i, +1, 0.0) for i in range(nW[1])]
+
+
+@pytest.mark.parametrize("nW,rank", [((0, 16), 1), ((8, 80), 2), ((3, 5), 3), ((4, 0), 1)])
+def test_parameter_mapping_matches_oracle(nW, rank):
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    L.init(wave_list(nW), rank)
+    Lo = O.OracleLikelihood(nW, rank)
+    assert L.nmbPars() == Lo.nPars
+    assert L.nmbWaves() == sum(nW) and L.nmbWaves(-1) == nW[0] and L.nmbWaves(+1) == nW[1]
+    rng = np.random.default_rng(1)
+    par = rng.normal(size=Lo.nPars)
+    pg, fg = L.copyFromParArray(par)
+    po, fo = Lo.copy_from_par(par)
+    assert np.array_equal(pg, po) and fg == fo
+    assert np.array_equal(L.copyToParArray(pg, fg), par)
+    # CorrectParamSigns on parameters with negative anchors
+    par[::3] *= -1
+    assert np.array_equal(L.CorrectParamSigns(par), Lo.correct_param_signs(par))
+
+
+def test_parameter_names():
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    L.init([("1-(1++)0+rho770_01_pi-", +1, 0.0), ("1-(0-+)0+f0980_00_pi-", +1, 0.0),
+            ("1-(1++)1-rho770_01_pi-", -1, 0.0)], 2)
+    names = [L.parameter(i)[0] for i in range(L.nmbPars())]
+    # pwaLikelihood.cc:107-122: V<rank>_<wave>_RE/_IM, V_flat_RE for the flat wave
+    assert names == ["V0_1-(1++)1-rho770_01_pi-_RE",
+                     "V0_1-(1++)0+rho770_01_pi-_RE", "V0_1-(0-+)0+f0980_00_pi-_RE", "V0_1-(0-+)0+f0980_00_pi-_IM",
+                     "V1_1-(0-+)0+f0980_00_pi-_RE",
+                     "V_flat_RE"]
+
+
+def test_thresholds_fix_parameters():
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    L.init([("a", -1, 0.0), ("b", -1, 1.0), ("c", +1, 0.0), ("d", +1, 0.0), ("e", +1, 2.0)], 1, massBinCenter=1.5)
+    fixed = [L.parameter(i)[2] for i in range(L.nmbPars())]
+    assert sum(fixed) == 2 and L.nmbParsFixed() == 2
+
+
+def test_call_order_errors():
+    from rootpwa_amd import capi
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    L.init(wave_list((1, 2)), 1)
+    m = PL.ampIntegralMatrix(["neg00", "pos00", "pos01"])
+    with pytest.raises(capi.RpwaError):
+        L.addAccIntegral(m)            # before addNormIntegral
+    with pytest.raises(capi.RpwaError):
+        L.addAmplitude("pos00", np.ones(3))  # before the integrals
+    with pytest.raises(capi.RpwaError) as e:
+        L.DoEval(np.zeros(L.nmbPars()))  # before finishInit
+    assert e.value.code == capi.RPWA_ERR_STATE
+    bad = PL.ampIntegralMatrix(["neg00", "pos00"])
+    with pytest.raises(capi.RpwaError):
+        L.addNormIntegral(bad)         # a wave of the list is missing
+
+
+def test_integral_matrix_bookkeeping(tmp_path):
+    from rootpwa_amd import likelihood as PL
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=(300, 4)) + 1j * rng.normal(size=(300, 4))
+    sums = O.integral(a)
+    m = PL.ampIntegralMatrix(["w0", "w1", "w2", "w3"])
+    m.setSums(sums, 300)
+    # element() divides component-wise (ampIntegralMatrix.cc:251, complex / double)
+    cdiv = lambda z, d: complex(z.real / d, z.imag / d)
+    assert m.element(1, 3) == cdiv(sums[1, 3], 300)
+    assert m.element("w2", "w0") == cdiv(sums[2, 0], 300)
+    f = str(tmp_path / "integral.txt")
+    m.writeAscii(f)
+    m2 = PL.ampIntegralMatrix()
+    m2.readAscii(f)
+    s2, n2 = m2.sums()
+    assert n2 == 300 and m2.waveNames() == ["w0", "w1", "w2", "w3"] and np.array_equal(s2, sums)
+    m2 += m
+    s3, n3 = m2.sums()
+    assert n3 == 600 and np.allclose(s3, 2 * sums, rtol=1e-15)
+    m2.renormalize(300)
+    s4, n4 = m2.sums()
+    assert n4 == 300 and np.allclose(s4, sums, rtol=1e-15)
+
+
+def test_norm_integral_reordering_against_oracle():
+    """addNormIntegral/addAccIntegral take elements by wave name: a permuted
+    integral matrix gives the same likelihood set-up."""
+    from rootpwa_amd import likelihood as PL
+    ds = H.dataset((2, 3), 1, 50, 500, seed=3)
+    names = [w[0] for w in wave_list((2, 3))]
+    perm = [4, 2, 0, 3, 1]
+    m = PL.ampIntegralMatrix([names[p] for p in perm])
+    m.setSums(ds["integral"][np.ix_(perm, perm)], ds["nMC"])
+    for i, a in enumerate(perm):
+        for j, b in enumerate(perm):
+            z = ds["integral"][a, b]
+            assert m.element(names[a], names[b]) == complex(z.real / ds["nMC"], z.imag / ds["nMC"])

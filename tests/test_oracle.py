@@ -1,0 +1,302 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each check fixes the oracle by something other than its own formula: the
+golden vectors computed in 60-digit decimal arithmetic (tests/golden), a
+closed form, mathematical identities (Euler homogeneity, phase and sign
+invariance, the extended-ML yield identity at a minimum), finite differences,
+an independent vectorised numpy formulation, and the reference's own
+compensated-sum scenario.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "likelihood_golden.json")
+
+
+def _cplx(a):
+    a = np.asarray(a)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def golden_cases():
+    with open(GOLDEN) as fh:
+        return json.load(fh)["cases"]
+
+
+def oracle_from_golden(c):
+    L = O.OracleLikelihood(tuple(c["nW"]), c["rank"])
+    if c["cauchy_width"] is not None:
+        L.set_prior(True, c["cauchy_width"])
+    L.add_integrals(_cplx(c["integral_sums"]), c["mc_events"])
+    L.add_amplitudes(_cplx(c["data"]))
+    L.finish_init()
+    return L
+
+
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
+def test_golden_vectors(name):
+    c = golden_cases()[name]
+    L = oracle_from_golden(c)
+    par = np.array(c["par"])
+    f, g = L.fdf(par)
+    assert abs(f - c["value"]) <= 1e-13 * max(1.0, abs(c["value"]))
+    gref = np.array(c["gradient"])
+    assert np.max(np.abs(g - gref)) <= 1e-13 * np.max(np.abs(gref))
+    assert L.doeval(par) == f
+    assert np.array_equal(L.gradient(par), g)
+    assert H.rel_err(O.integral(_cplx(c["mc"])), _cplx(c["integral_sums"])) <= 1e-14
+
+
+def test_kahan_scenario_of_the_reference():
+    # test/utilities/testSumAccumulators.cc:46-60: 1e6 additions of 1e-6 in float
+    naive, comp = O.kahan_float_demo(1000000, 1e-6)
+    assert comp == pytest.approx(1.0, abs=1e-7)
+    assert abs(naive - 1.0) > 1e-3
+
+
+@pytest.mark.parametrize("nW,rank,P", [((0, 16), 1, 32), ((8, 80), 2, 345), ((0, 88), 1, 176), ((7, 34), 2, 157),
+                                       ((3, 3), 4, 19)])
+def test_parameter_counts(nW, rank, P):
+    L = O.OracleLikelihood(nW, rank)
+    assert L.nPars == P
+    # rank restrictions: zero below the diagonal, real anchor, flat last
+    m = L.par_map()
+    for r in range(rank):
+        for s in range(2):
+            for i in range(nW[s]):
+                if i < r:
+                    assert tuple(m[r, s, i]) == (-1, -1)
+                elif i == r:
+                    assert m[r, s, i, 0] >= 0 and m[r, s, i, 1] == -1
+                else:
+                    assert m[r, s, i, 1] == m[r, s, i, 0] + 1
+    assert L.par_info(P - 1)[1] == -1  # flat wave
+    rng = np.random.default_rng(0)
+    par = rng.normal(size=P)
+    prod, flat = L.copy_from_par(par)
+    assert np.array_equal(L.copy_to_par(prod, flat), par)
+
+
+def test_thresholds_fix_parameters():
+    nW = (2, 3)
+    thr = np.zeros((2, 3))
+    thr[1, 2] = 2.0   # above the bin centre -> fixed
+    thr[0, 1] = 1.0   # below -> free
+    L = O.OracleLikelihood(nW, 1, thresholds=thr, massBinCenter=1.5)
+    fixed = [L.par_info(i)[3] for i in range(L.nPars)]
+    assert sum(fixed) == 2 and L.nmb_pars_fixed() == 2
+
+
+def test_closed_form_one_wave():
+    # one wave, rank 1: -lnL = -sum ln(T^2 |A|^2 + f^2) + (T^2 I + f^2 eta)
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=50) + 1j * rng.normal(size=50)
+    mc = rng.normal(size=400) + 1j * rng.normal(size=400)
+    Isum = np.sum(np.abs(mc) ** 2)
+    L = O.OracleLikelihood((0, 1), 1)
+    L.add_integrals(np.array([[Isum]]), 400)
+    L.add_amplitudes(a[:, None])
+    L.finish_init()
+    T, f = 0.7, 0.25
+    An = a / np.sqrt(Isum / 400)
+    expect = -np.sum(np.log(T * T * np.abs(An) ** 2 + f * f)) + (T * T + f * f)
+    assert L.doeval(np.array([T, f])) == pytest.approx(expect, rel=1e-14)
+    # non-normalised amplitudes: nu = N and no rescaling
+    L2 = O.OracleLikelihood((0, 1), 1)
+    L2.use_normalized_amps(False)
+    L2.add_integrals(np.array([[Isum]]), 400)
+    L2.add_amplitudes(a[:, None])
+    L2.finish_init()
+    expect2 = -np.sum(np.log(T * T * np.abs(a) ** 2 + f * f)) + 50 * (T * T * Isum / 400 + f * f)
+    assert L2.doeval(np.array([T, f])) == pytest.approx(expect2, rel=1e-14)
+
+
+def numpy_neg_lnl(L, prod, flat):
+    """Independent vectorised formulation (einsum over the sector blocks)."""
+    A = [L.decay_amps(0), L.decay_amps(1)]
+    Iacc = L.acc_matrix()
+    sigma = np.full(L.nmb_events(), flat * flat)
+    norm = flat * flat * L.tot_acc()
+    for s in range(2):
+        n = L.nW[s]
+        if n == 0:
+            continue
+        S = np.einsum("ri,ei->er", prod[:, s, :n], A[s])
+        sigma += np.sum(np.abs(S) ** 2, axis=1)
+        norm += np.real(np.einsum("ri,rj,ij->", prod[:, s, :n], np.conj(prod[:, s, :n]), Iacc[s, :n, s, :n]))
+    return -np.sum(np.log(sigma)) + norm
+
+
+@pytest.fixture(scope="module")
+def small():
+    ds = H.dataset((3, 5), 2, 400, 2000, seed=21)
+    return ds, H.oracle_likelihood(ds)
+
+
+def test_matches_vectorised_formulation(small):
+    ds, L = small
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        par = rng.normal(size=L.nPars)
+        prod, flat = L.copy_from_par(par)
+        assert L.doeval(par) == pytest.approx(numpy_neg_lnl(L, prod, flat), rel=1e-12)
+
+
+def test_gradient_finite_differences(small):
+    ds, L = small
+    rng = np.random.default_rng(3)
+    par = rng.normal(size=L.nPars)
+    g = L.gradient(par)
+    h = 1e-6
+    num = np.array([(L.doeval(par + h * e) - L.doeval(par - h * e)) / (2 * h) for e in np.eye(L.nPars)])
+    assert np.max(np.abs(num - g)) <= 1e-6 * np.max(np.abs(g))
+
+
+def test_euler_identity(small):
+    # sigma_e and the normalisation term are homogeneous of degree 2:
+    # sum_k par_k dL/dpar_k = -2N + 2 nu NormTerm
+    ds, L = small
+    rng = np.random.default_rng(4)
+    par = rng.normal(size=L.nPars)
+    f, g = L.fdf(par)
+    prod, flat = L.copy_from_par(par)
+    event = L.event_loglik(prod, flat)
+    norm = f - event
+    assert np.dot(par, g) == pytest.approx(-2 * L.nmb_events() + 2 * norm, rel=1e-11)
+
+
+def test_phase_and_sign_invariance(small):
+    ds, L = small
+    rng = np.random.default_rng(5)
+    par = rng.normal(size=L.nPars)
+    prod, flat = L.copy_from_par(par)
+    f0 = L.event_loglik(prod, flat)
+    for r in range(2):
+        for s in range(2):
+            p = prod.copy()
+            p[r, s] *= np.exp(1j * 0.7)
+            assert L.event_loglik(p, flat) == pytest.approx(f0, rel=1e-13)
+    # CorrectParamSigns flips (rank, sector) blocks whose real anchor is negative: bitwise equal
+    q = L.correct_param_signs(par)
+    pq, fq = L.copy_from_par(q)
+    assert fq >= 0
+    for r in range(2):
+        for s in range(2):
+            assert not (pq[r, s, r].real < 0 and pq[r, s, r].imag == 0)
+    assert L.doeval(q) == L.doeval(par)
+
+
+def test_doderivative_cache(small):
+    ds, L = small
+    par = np.random.default_rng(6).normal(size=L.nPars)
+    f, g = L.fdf(par)
+    assert all(L.doderivative(par, k) == g[k] for k in range(0, L.nPars, 7))
+    par2 = par + 0.01
+    g2 = L.gradient(par2)
+    assert L.doderivative(par2, 3) == g2[3]
+
+
+def test_half_cauchy_prior_closed_form(small):
+    ds, _ = small
+    L = H.oracle_likelihood(ds, prior=0.6)
+    Lf = H.oracle_likelihood(ds)
+    par = np.random.default_rng(7).normal(size=L.nPars)
+    prod, flat = L.copy_from_par(par)
+    r2 = np.abs(prod[prod != 0]) ** 2
+    prior = np.sum(np.log(1 + r2 / 0.36))
+    assert L.doeval(par) - Lf.doeval(par) == pytest.approx(prior, rel=1e-12)
+    g = L.gradient(par)
+    h = 1e-6
+    num = np.array([(L.doeval(par + h * e) - L.doeval(par - h * e)) / (2 * h) for e in np.eye(L.nPars)])
+    assert np.max(np.abs(num - g)) <= 1e-6 * np.max(np.abs(g))
+
+
+def test_hessian_finite_differences(small):
+    ds, L = small
+    for prior in (None, 0.5):
+        Lp = H.oracle_likelihood(ds, prior=prior) if prior else L
+        par = np.random.default_rng(8).normal(size=Lp.nPars)
+        Hm = Lp.hessian(par)
+        assert np.allclose(Hm, Hm.T, rtol=1e-10, atol=1e-9 * np.max(np.abs(Hm)))
+        h = 1e-5
+        num = np.array([(Lp.gradient(par + h * e) - Lp.gradient(par - h * e)) / (2 * h) for e in np.eye(Lp.nPars)])
+        assert np.max(np.abs(num - Hm)) <= 1e-5 * np.max(np.abs(Hm))
+
+
+def test_yield_identity_at_minimum():
+    # at an extended-ML minimum with normalised amplitudes, NormTerm = N
+    from scipy.optimize import minimize
+    ds = H.dataset((0, 4), 1, 600, 3000, seed=31)
+    L = H.oracle_likelihood(ds)
+    x0 = np.full(L.nPars, 3.0)
+    res = minimize(lambda p: L.fdf(p), x0, jac=True, method="L-BFGS-B", options=dict(gtol=1e-9, maxiter=2000))
+    x = res.x
+    for _ in range(6):  # Newton polish with the analytic Hessian
+        x = x - np.linalg.solve(L.hessian(x), L.gradient(x))
+    f, g = L.fdf(x)
+    assert np.max(np.abs(g)) < 1e-8
+    prod, flat = L.copy_from_par(x)
+    norm = f - L.event_loglik(prod, flat)
+    assert norm == pytest.approx(L.nmb_events(), rel=1e-10)
+
+
+def test_integral_properties():
+    rng = np.random.default_rng(9)
+    a = rng.normal(size=(500, 6)) + 1j * rng.normal(size=(500, 6))
+    I = O.integral(a)
+    assert np.allclose(I, I.conj().T, rtol=0, atol=1e-12 * np.max(np.abs(I)))
+    assert np.all(np.diag(I).real > 0) and np.allclose(np.diag(I).imag, 0)
+    assert np.min(np.linalg.eigvalsh(I)) > 0
+    # I_ij = sum_e A_i conj(A_j)
+    assert H.rel_err(I, a.T @ np.conj(a)) <= 1e-13
+    # weights 1 reduce to the plain sum; constant weights cancel in the renormalisation
+    assert H.rel_err(O.integral(a, np.ones(500)), I) <= 1e-14
+    assert H.rel_err(O.integral(a, np.full(500, 3.0)), I) <= 1e-14
+    # additive over event subsets
+    assert H.rel_err(O.integral(a[:200]) + O.integral(a[200:]), I) <= 1e-13
+    # importance weights: sum (1/w) A A^H renormalised by mean(1/w)
+    w = rng.uniform(0.5, 2, size=500)
+    ref = (a.T * (1 / w)) @ np.conj(a) / np.mean(1 / w)
+    assert H.rel_err(O.integral(a, w), ref) <= 1e-13
+
+
+def test_zero_wave_is_fixed():
+    # all-zero amplitudes of a wave whose acceptance integral is zero fix its parameters
+    # (pwaLikelihood.cc:1248-1282), without changing nmbParsFixed
+    rng = np.random.default_rng(10)
+    a = rng.normal(size=(50, 3)) + 1j * rng.normal(size=(50, 3))
+    a[:, 2] = 0
+    mc = rng.normal(size=(300, 3)) + 1j * rng.normal(size=(300, 3))
+    mc[:, 2] = 0
+    I = O.integral(mc)
+    I[2, 2] = 1.0  # normalisation integral non-zero, acceptance zero
+    acc = O.integral(mc)
+    L = O.OracleLikelihood((0, 3), 1)
+    L.add_integrals(I, 300, acc, 300)
+    L.add_amplitudes(a)
+    fixed = [L.par_info(i)[3] for i in range(L.nPars)]
+    assert fixed == [0, 0, 0, 1, 1, 0] and L.nmb_pars_fixed() == 0
+
+
+def test_generator_restatement():
+    seed = 5
+    s = O.gen_wave_scales(seed, 1000)
+    assert np.all((s >= 0.1) & (s <= 1.0)) and abs(np.mean(s) - 0.55) < 0.02
+    a = O.gen_amps(seed, 3, 0, 20000, np.ones(4))
+    # lognormal(0, 0.5) per event times complex normals: E|A|^2 = 2 E[exp(N(0,1))] = 2 e^0.5
+    assert np.mean(np.abs(a) ** 2) == pytest.approx(2 * np.exp(0.5), rel=0.03)
+    # counter-based: any sub-range regenerates identically
+    assert np.array_equal(O.gen_amps(seed, 3, 500, 10, np.ones(4)), a[500:510])
+    # accept-reject keeps pool order and draws events proportional to the intensity
+    prod = O.gen_prod_amps(seed, 2, 1, (0, 4))
+    data, idx, pool, smax = O.gen_accept_reject(seed, 4, 300, (0, 4), 1, prod, np.ones(4))
+    assert np.all(np.diff(idx.astype(np.int64)) > 0)
+    sig = np.abs(data @ prod[0, 1]) ** 2
+    pool_a = O.gen_amps(seed, 4, 0, 20000, np.ones(4))
+    assert np.mean(sig) > 1.5 * np.mean(np.abs(pool_a @ prod[0, 1]) ** 2)
