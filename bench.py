@@ -72,6 +72,9 @@ def main():
     dev = capi.device_info(0)
     log("device", dev)
 
+    read_gbs, copy_gbs = capi.bandwidth_probe(4 << 30, 10)
+    log("bandwidth probe: read %.0f GB/s, copy %.0f GB/s (4 GiB)" % (read_gbs, copy_gbs))
+
     nW = (8, 80)
     W = sum(nW)
     rank = 2
@@ -216,12 +219,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach_mix, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_mix / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "event_kernel<mixed> (fused lnL + gradient)",
-                     "kernel_ms": 1e3 * r_mix["kernel_s"], "bytes_per_launch": bytes_mix},
+                     "kernel_ms": 1e3 * r_mix["kernel_s"], "bytes_per_launch": bytes_mix,
+                     "measured_read_GBs": read_gbs, "frac_of_measured_read": ach_mix / read_gbs},
         "fp64": {"evals_per_s": r_64["rate"], "ms_per_eval": r_64["ms"], "kernel_ms": 1e3 * r_64["kernel_s"],
-                 "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS},
+                 "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs},
         "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
         "integral": {"ms": 1e3 * t_int, "tflops_algorithmic": int_flops / t_int / 1e12,
                      "frac_fp64_mfma": int_flops / t_int / 1e12 / FP64_MFMA_PEAK_TFS, "mc_events": M},
+        "bandwidth_probe": {"read_GBs": read_gbs, "copy_GBs": copy_gbs, "bytes": 4 << 30},
         "check": check,
         "cpu_baseline": cpu,
     }

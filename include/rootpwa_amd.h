@@ -161,6 +161,11 @@ int rpwa_dev_synchronize(int device);
  * bytes, peak engine clock [kHz] */
 int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* totalMem, int* clockKhz);
 
+/* HBM bandwidth probe: median GB/s of a streaming read and of a copy (read +
+ * write bytes) over `bytes` of device memory; the counterpart of the
+ * reference's bandwidth microbenchmarks (cuda/test/benchmarkMemoryBandwidth.cu). */
+int rpwa_bandwidth_probe(int device, uint64_t bytes, int repetitions, double* readGBs, double* copyGBs);
+
 /* ---------------------------------------------------------------------------
  * Normalisation integrals: ampIntegralMatrix::integrate
  * (decayAmplitude/ampIntegralMatrix.cc:300-488) for amplitudes with one

@@ -63,6 +63,7 @@ PROTOTYPES = {
     "rpwa_dev_copy": (_i, [_vp, _vp, _u64, _i]),
     "rpwa_dev_synchronize": (_i, [_i]),
     "rpwa_dev_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i), _u64p, ctypes.POINTER(_i)]),
+    "rpwa_bandwidth_probe": (_i, [_i, _u64, _i, _dp, _dp]),
     "rpwa_integral": (_i, [_i, _dp, _u64, _u32, _dp, _dp]),
     "rpwa_integral_device": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
     "rpwa_gen_wave_scales": (_i, [_u64, _u32, _dp]),
@@ -292,6 +293,13 @@ def device_info(device=0):
     clk = ctypes.c_int()
     call("rpwa_dev_info", int(device), name, 64, ctypes.byref(cu), ctypes.byref(mem), ctypes.byref(clk))
     return dict(arch=name.value.decode(), cus=cu.value, memory=mem.value, clock_khz=clk.value)
+
+
+def bandwidth_probe(nbytes=4 << 30, repetitions=10, device=0):
+    """(read GB/s, copy GB/s) of streaming kernels over nbytes of HBM."""
+    r, c = ctypes.c_double(), ctypes.c_double()
+    call("rpwa_bandwidth_probe", int(device), int(nbytes), int(repetitions), ctypes.byref(r), ctypes.byref(c))
+    return r.value, c.value
 
 
 def integral(amps, weights=None, device=0):
