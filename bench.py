@@ -38,6 +38,19 @@ def wave_list(nW):
     return [("neg%02d" % i, -1, 0.0) for i in range(nW[0])] + [("pos%02d" % i, +1, 0.0) for i in range(nW[1])]
 
 
+def measured_traffic(N, W):
+    """HBM bytes per launch of the event kernels from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel),
+    valid for the default workload only."""
+    path = os.path.join(ROOT, "profiles", "event_kernel_2Mx88_pmc.txt")
+    if (N, W) != (2_000_000, 88) or not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+            return {k: float(v) for k, v in (kv.split("=") for kv in line.split()[1:])}
+    return {}
+
+
 def start_values(rng, nPars, nEvents, W):
     """Random start at the scale of the fit (|T|^2 ~ N/W per wave)."""
     return rng.normal(size=nPars) * np.sqrt(nEvents / W)
@@ -196,6 +209,7 @@ def main():
     bytes_64 = float(N) * W * 16
     ach_mix = bytes_mix / r_mix["kernel_s"] / 1e9
     ach_64 = bytes_64 / r_64["kernel_s"] / 1e9
+    traffic = measured_traffic(N, W)
     out = {
         "metric": METRIC,
         "value": r_mix["rate"],
@@ -217,12 +231,13 @@ def main():
             "data_pool": pool, "seed": SEED, "device": dev["arch"], "cus": dev["cus"],
         },
         "roofline": {"bound": "hbm", "achieved": ach_mix, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_mix / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_mix / HBM_PEAK_GBS, "traffic": traffic.get("mixed"),
                      "kernel": "event_kernel<mixed> (fused lnL + gradient)",
                      "kernel_ms": 1e3 * r_mix["kernel_s"], "bytes_per_launch": bytes_mix,
                      "measured_read_GBs": read_gbs, "frac_of_measured_read": ach_mix / read_gbs},
         "fp64": {"evals_per_s": r_64["rate"], "ms_per_eval": r_64["ms"], "kernel_ms": 1e3 * r_64["kernel_s"],
-                 "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs},
+                 "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs,
+                 "traffic": traffic.get("fp64")},
         "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
         "integral": {"ms": 1e3 * t_int, "tflops_algorithmic": int_flops / t_int / 1e12,
                      "frac_fp64_mfma": int_flops / t_int / 1e12 / FP64_MFMA_PEAK_TFS, "mc_events": M},
