@@ -712,19 +712,6 @@ extern "C" int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream)
 	return RPWA_OK;
 }
 
-extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
-                                double prodAmpFlat, uint32_t rank, double* hessian, double* flatTerms,
-                                double* hessianFlat)
-{
-	clear_error();
-	if (int st = check_ctx(ctx, "rpwa_lik_hessian"))
-		return st;
-	(void)prodAmps; (void)nmbProdAmps; (void)prodAmpFlat; (void)rank;
-	(void)hessian; (void)flatTerms; (void)hessianFlat;
-	set_error("rpwa_lik_hessian: not built yet");
-	return RPWA_ERR_UNSUPPORTED;
-}
-
 extern "C" int rpwa_dev_alloc(int device, uint64_t bytes, void** ptr)
 {
 	clear_error();

@@ -1,0 +1,219 @@
+// hessian.hip -- event part of the analytic Hessian on the GPU.
+//
+// pwaLikelihood::Hessian (partialWaveFit/pwaLikelihood.cc:697-753) sums per event
+//   factor2 * Re/Im(d_b) * Re/Im(d_a)            with d_a = S_{r_a s_a}(e) conj(A_a(e)),
+//   - factor * Re/Im(A_a conj(A_b))               for a, b of the same (rank, sector),
+//   factor2 * flat * d_a  and  factor2 * flat^2 - factor,
+// factor = 2 / sigma_e.  Each is a weighted rank-k update over events: a row
+// kernel writes, per event, the vector (d_a..., 1) with weight factor^2 and
+// the amplitude row (A..., 1) with weight -factor; the FP64-MFMA rank-k update
+// of integral.hip sums x x^T for both; the constant column yields the flat-wave
+// terms.  Events are processed in chunks so the row buffers stay bounded.
+#include "internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace rpwa_amd {
+namespace {
+
+template <bool MIXED>
+__device__ __forceinline__ double2 amp_at(const void* amps, uint64_t e, uint32_t rowLen, uint32_t pos)
+{
+	if (MIXED) {
+		const float2 v = reinterpret_cast<const float2*>(amps)[e * rowLen + pos];
+		return make_double2((double)v.x, (double)v.y);
+	}
+	return reinterpret_cast<const double2*>(amps)[e * rowLen + pos];
+}
+
+template <bool MIXED>
+__global__ void hessian_rows_kernel(const void* amps, uint64_t e0, uint64_t n, uint32_t rowLen, uint32_t secOff1,
+                                    uint32_t nW0, uint32_t nW1, uint32_t wmax, int rank,
+                                    const double2* __restrict__ prod, double flat2, uint32_t nd,
+                                    double2* __restrict__ dRows, double2* __restrict__ aRows,
+                                    double* __restrict__ wD, double* __restrict__ wA)
+{
+	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const uint64_t e = e0 + k;
+	const uint32_t W = nW0 + nW1;
+	double2 S[kMaxRank][2];
+	double sigma = 0.;
+	for (int r = 0; r < kMaxRank; ++r)
+		for (int s = 0; s < 2; ++s) {
+			double2 acc = make_double2(0., 0.);
+			const uint32_t nw = s ? nW1 : nW0;
+			if (r < rank)
+				for (uint32_t i = 0; i < nw; ++i) {
+					const double2 t = prod[((size_t)r * 2 + s) * wmax + i];
+					const double2 x = amp_at<MIXED>(amps, e, rowLen, (s ? secOff1 : 0) + i);
+					acc.x = fma(t.x, x.x, acc.x);
+					acc.x = fma(-t.y, x.y, acc.x);
+					acc.y = fma(t.x, x.y, acc.y);
+					acc.y = fma(t.y, x.x, acc.y);
+				}
+			S[r][s] = acc;
+			sigma += acc.x * acc.x + acc.y * acc.y;
+		}
+	sigma += flat2;
+	const double factor = 2. / sigma;
+	wD[k] = factor * factor;
+	wA[k] = -factor;
+	double2* drow = dRows + k * (nd + 1);
+	uint32_t q = 0;
+	for (int r = 0; r < kMaxRank; ++r) {
+		if (r >= rank)
+			break;
+		for (int s = 0; s < 2; ++s) {
+			const uint32_t nw = s ? nW1 : nW0;
+			for (uint32_t i = 0; i < nw; ++i) {
+				const double2 x = amp_at<MIXED>(amps, e, rowLen, (s ? secOff1 : 0) + i);
+				// d = S conj(A)
+				drow[q++] = make_double2(S[r][s].x * x.x + S[r][s].y * x.y, S[r][s].y * x.x - S[r][s].x * x.y);
+			}
+		}
+	}
+	drow[nd] = make_double2(1., 0.);
+	double2* arow = aRows + k * (W + 1);
+	for (uint32_t i = 0; i < W; ++i)
+		arow[i] = amp_at<MIXED>(amps, e, rowLen, i < nW0 ? i : secOff1 + (i - nW0));
+	arow[W] = make_double2(1., 0.);
+}
+
+}  // namespace
+}  // namespace rpwa_amd
+
+using namespace rpwa_amd;
+
+extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
+                                double prodAmpFlat, uint32_t rank, double* hessian, double* flatTerms,
+                                double* hessianFlat)
+{
+	clear_error();
+	if (!ctx) {
+		set_error("rpwa_lik_hessian: null context");
+		return RPWA_ERR_INVALID;
+	}
+	if (!ctx->loaded) {
+		set_error("rpwa_lik_hessian: decay amplitudes not loaded");
+		return RPWA_ERR_STATE;
+	}
+	if (!prodAmps || !hessian || !flatTerms || !hessianFlat || rank == 0 || rank > (uint32_t)kMaxRank) {
+		set_error("rpwa_lik_hessian: null output or rank outside 1.." + std::to_string(kMaxRank));
+		return RPWA_ERR_INVALID;
+	}
+	const AmpLayout& lay = ctx->lay;
+	const uint32_t wmax = lay.wmax, W = lay.nW[0] + lay.nW[1];
+	if (nmbProdAmps != rank * 2 * wmax) {
+		set_error("rpwa_lik_hessian: nmbProdAmps must be rank*2*nmbWavesMax");
+		return RPWA_ERR_INVALID;
+	}
+	try {
+		RPWA_HIP_CHECK(hipSetDevice(ctx->device));
+		const uint32_t nd = rank * W;  // valid (rank, sector, wave) entries
+		const uint32_t dD = 2 * (nd + 1), dA = 2 * (W + 1);
+		// event chunk: row buffers of at most ~2 GiB
+		const uint64_t chunk = std::max<uint64_t>(1024, (2ull << 30) / (16ull * (nd + 1)));
+		const uint64_t cn = std::min<uint64_t>(chunk, lay.nEvents);
+		double2 *dRows = nullptr, *aRows = nullptr, *prod = nullptr;
+		double *wD = nullptr, *wA = nullptr;
+		auto cleanup = [&]() {
+			hipFree(dRows); hipFree(aRows); hipFree(prod); hipFree(wD); hipFree(wA);
+		};
+		if (hipMalloc(&dRows, cn * (nd + 1) * sizeof(double2)) != hipSuccess ||
+		    hipMalloc(&aRows, cn * (W + 1) * sizeof(double2)) != hipSuccess ||
+		    hipMalloc(&prod, (size_t)nmbProdAmps * sizeof(double2)) != hipSuccess ||
+		    hipMalloc(&wD, cn * sizeof(double)) != hipSuccess || hipMalloc(&wA, cn * sizeof(double)) != hipSuccess) {
+			cleanup();
+			set_error("rpwa_lik_hessian: device allocation failed");
+			return RPWA_ERR_NOMEM;
+		}
+		hipError_t e = hipMemcpy(prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2), hipMemcpyHostToDevice);
+		std::vector<double> CD((size_t)dD * dD, 0.), CA((size_t)dA * dA, 0.), part;
+		int st = RPWA_OK;
+		for (uint64_t e0 = 0; e == hipSuccess && st == RPWA_OK && e0 < lay.nEvents; e0 += cn) {
+			const uint64_t n = std::min<uint64_t>(cn, lay.nEvents - e0);
+			const uint32_t nb = (uint32_t)((n + 255) / 256);
+			if (lay.precision == RPWA_PREC_MIXED)
+				hipLaunchKernelGGL(hessian_rows_kernel<true>, dim3(nb), dim3(256), 0, 0, ctx->d_amps, e0, n,
+				                   lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
+				                   prodAmpFlat * prodAmpFlat, nd, dRows, aRows, wD, wA);
+			else
+				hipLaunchKernelGGL(hessian_rows_kernel<false>, dim3(nb), dim3(256), 0, 0, ctx->d_amps, e0, n,
+				                   lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
+				                   prodAmpFlat * prodAmpFlat, nd, dRows, aRows, wD, wA);
+			e = hipGetLastError();
+			if (e == hipSuccess)
+				e = hipDeviceSynchronize();
+			if (e != hipSuccess)
+				break;
+			st = syrk_run(ctx->device, reinterpret_cast<const double*>(dRows), n, dD, wD, true, nullptr, nullptr,
+			              part, nullptr);
+			if (st == RPWA_OK) {
+				for (size_t i = 0; i < CD.size(); ++i)
+					CD[i] += part[i];
+				st = syrk_run(ctx->device, reinterpret_cast<const double*>(aRows), n, dA, wA, true, nullptr,
+				              nullptr, part, nullptr);
+				if (st == RPWA_OK)
+					for (size_t i = 0; i < CA.size(); ++i)
+						CA[i] += part[i];
+			}
+		}
+		cleanup();
+		RPWA_HIP_CHECK(e);
+		if (st != RPWA_OK)
+			return st;
+		// assemble [rank][2][wmax] x [rank][2][wmax] x 3 (pwaLikelihood.cc:697-753)
+		const size_t ndFull = (size_t)rank * 2 * wmax;
+		std::vector<int> valid(ndFull, -1), gwave(ndFull, 0);
+		{
+			uint32_t q = 0;
+			for (uint32_t r = 0; r < rank; ++r)
+				for (uint32_t s = 0; s < 2; ++s)
+					for (uint32_t i = 0; i < lay.nW[s]; ++i) {
+						const size_t a = ((size_t)r * 2 + s) * wmax + i;
+						valid[a] = (int)q++;
+						gwave[a] = (int)(s ? lay.nW[0] + i : i);
+					}
+		}
+		auto cd = [&](size_t r, size_t c) { return CD[r * dD + c]; };
+		auto ca = [&](size_t r, size_t c) { return CA[r * dA + c]; };
+		std::memset(hessian, 0, ndFull * ndFull * 3 * sizeof(double));
+		for (size_t a = 0; a < ndFull; ++a) {
+			const int va = valid[a];
+			if (va < 0) {
+				flatTerms[2 * a] = flatTerms[2 * a + 1] = 0.;
+				continue;
+			}
+			const size_t ra = a / (2 * wmax), sa = (a / wmax) % 2;
+			for (size_t b = 0; b < ndFull; ++b) {
+				const int vb = valid[b];
+				if (vb < 0)
+					continue;
+				const size_t rb = b / (2 * wmax), sb = (b / wmax) % 2;
+				double* h = hessian + (a * ndFull + b) * 3;
+				h[0] = cd(2 * va, 2 * vb);
+				h[1] = cd(2 * vb + 1, 2 * va);
+				h[2] = cd(2 * va + 1, 2 * vb + 1);
+				if (ra == rb && sa == sb) {
+					const size_t ga = gwave[a], gb = gwave[b];
+					const double re = ca(2 * ga, 2 * gb) + ca(2 * ga + 1, 2 * gb + 1);
+					const double im = ca(2 * ga + 1, 2 * gb) - ca(2 * ga, 2 * gb + 1);
+					h[0] += re;
+					h[1] += im;
+					h[2] += re;
+				}
+			}
+			flatTerms[2 * a] = prodAmpFlat * cd(2 * va, 2 * nd);
+			flatTerms[2 * a + 1] = prodAmpFlat * cd(2 * va + 1, 2 * nd);
+		}
+		*hessianFlat = prodAmpFlat * prodAmpFlat * cd(2 * nd, 2 * nd) + ca(2 * W, 2 * W);
+		return RPWA_OK;
+	} catch (const std::exception& ex) {
+		set_error(std::string("rpwa_lik_hessian: ") + ex.what());
+		return RPWA_ERR_NOMEM;
+	}
+}

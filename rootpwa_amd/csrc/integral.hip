@@ -27,9 +27,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIntThreads = 256;
 constexpr int kIntWaves = 4;
-constexpr int kChunk = 32;         // events per staged chunk (8 MFMA k-steps)
 constexpr int kMaxTilesPerWave = 24;
-constexpr int kMaxLoadsPerThread = 16;  // double2 loads per thread and chunk (D <= 256)
+constexpr int kMaxLoadsPerThread = 16;  // double2 loads per thread and chunk: CH * D / 2 <= 4096
 
 struct IntArgs {
 	const double* x;       // [M][D] row-major (interleaved complex rows)
@@ -45,6 +44,7 @@ struct IntArgs {
 	double* partials;      // [nWg][nTiles][256]
 	double* wsum;          // [nWg] sum of weights (1/w), or null
 	int accumulate;        // add to the existing partials
+	int directWeights;     // weight = w[e] instead of 1/w[e]
 };
 
 __device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, uint32_t& J)
@@ -63,7 +63,7 @@ __device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, ui
 // pass; slots past the end of the pass compute a dummy tile that is discarded,
 // so the MFMA loop has no data-dependent exits and the accumulators stay in
 // registers.
-template <int NT>
+template <int NT, int kChunk>
 __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 {
 	extern __shared__ double lds[];   // 2 x ([kChunk][ldsStride] + weights[kChunk])
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 		}
 		if (tid < kChunk) {
 			const uint64_t e = e0 + tid;
-			wstage = e < a.M ? (a.w ? 1. / a.w[e] : 1.) : 0.;
+			wstage = e < a.M ? (a.w ? (a.directWeights ? a.w[e] : 1. / a.w[e]) : 1.) : 0.;
 		}
 	};
 	auto store_chunk = [&](int buf) {
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 			o[i * 16 + j] = val;
 		}
 	}
-	__shared__ double wsh[kChunk];
+	__shared__ double wsh[32];
 	if (tid < kChunk)
 		wsh[tid] = wacc;
 	__syncthreads();
@@ -229,24 +229,29 @@ __global__ __launch_bounds__(kColElems* kColChunks) void column_sum_kernel(
 
 struct IntegralPlan {
 	uint32_t D, Dp, nT, nTiles, ldsStride, tilesPerPass;
+	int chunk;
 	int nWg;
 	size_t lds;
 };
 
-static int integral_plan(int device, uint32_t nWaves, IntegralPlan& p)
+// D = real row length (2 x complex values per row)
+static int integral_plan(int device, uint32_t D, IntegralPlan& p)
 {
-	p.D = 2 * nWaves;
+	p.D = D;
 	p.Dp = (p.D + 15) / 16 * 16;
 	p.nT = p.Dp / 16;
 	p.nTiles = p.nT * (p.nT + 1) / 2;
 	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
 	p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
-	p.lds = 2 * ((size_t)kChunk * p.ldsStride + kChunk) * sizeof(double);
-	if ((size_t)kChunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
-		set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / kChunk) +
-		          " waves per amplitude row");
+	p.chunk = 32;
+	while (p.chunk > 4 && (size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread)
+		p.chunk /= 2;
+	if ((size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
+		set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / 8) +
+		          " complex values per row");
 		return RPWA_ERR_UNSUPPORTED;
 	}
+	p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
 	int nCu = 0;
 	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
 	p.nWg = nCu > 0 ? nCu : 1;
@@ -254,50 +259,49 @@ static int integral_plan(int device, uint32_t nWaves, IntegralPlan& p)
 }
 
 typedef void (*IntKernel)(IntArgs);
-static IntKernel pick_int_kernel(uint32_t nTiles)
+template <int CH>
+static IntKernel pick_int_nt(uint32_t per)
+{
+	if (per <= 1) return integral_kernel<1, CH>;
+	if (per <= 2) return integral_kernel<2, CH>;
+	if (per <= 4) return integral_kernel<4, CH>;
+	if (per <= 6) return integral_kernel<6, CH>;
+	if (per <= 9) return integral_kernel<9, CH>;
+	if (per <= 12) return integral_kernel<12, CH>;
+	if (per <= 17) return integral_kernel<17, CH>;
+	return integral_kernel<24, CH>;
+}
+
+static IntKernel pick_int_kernel(uint32_t nTiles, int chunk)
 {
 	const uint32_t per = (nTiles + kIntWaves - 1) / kIntWaves;
-	if (per <= 1) return integral_kernel<1>;
-	if (per <= 2) return integral_kernel<2>;
-	if (per <= 4) return integral_kernel<4>;
-	if (per <= 6) return integral_kernel<6>;
-	if (per <= 9) return integral_kernel<9>;
-	if (per <= 12) return integral_kernel<12>;
-	if (per <= 17) return integral_kernel<17>;
-	return integral_kernel<24>;
+	if (chunk >= 32) return pick_int_nt<32>(per);
+	if (chunk >= 16) return pick_int_nt<16>(per);
+	if (chunk >= 8) return pick_int_nt<8>(per);
+	return pick_int_nt<4>(per);
 }
 
-// assemble the complex [W][W] sums from the reduced upper-triangle tiles
-static void assemble(const std::vector<double>& C, const IntegralPlan& p, uint32_t W, double* out)
+static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_t r, uint32_t c)
 {
-	auto tileIndex = [&](uint32_t I, uint32_t J) {
-		// row-by-row enumeration of the upper triangle
-		return I * p.nT - I * (I - 1) / 2 + (J - I);
-	};
-	auto getC = [&](uint32_t r, uint32_t c) {
-		uint32_t I = r / 16, J = c / 16;
-		if (I > J) {
-			std::swap(r, c);
-			std::swap(I, J);
-		}
-		return C[(size_t)tileIndex(I, J) * 256 + (r % 16) * 16 + (c % 16)];
-	};
-	for (uint32_t i = 0; i < W; ++i)
-		for (uint32_t j = 0; j < W; ++j) {
-			const double re = getC(2 * i, 2 * j) + getC(2 * i + 1, 2 * j + 1);
-			const double im = getC(2 * i + 1, 2 * j) - getC(2 * i, 2 * j + 1);
-			out[2 * ((size_t)i * W + j)] = re;
-			out[2 * ((size_t)i * W + j) + 1] = im;
-		}
+	uint32_t I = r / 16, J = c / 16;
+	if (I > J) {
+		std::swap(r, c);
+		std::swap(I, J);
+	}
+	// row-by-row enumeration of the upper triangle
+	const uint32_t t = I * p.nT - I * (I - 1) / 2 + (J - I);
+	return C[(size_t)t * 256 + (r % 16) * 16 + (c % 16)];
 }
 
-static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W,
-                        const double* d_w, double* outSums, const double* h_amps,
-                        const double* h_w)
+// Weighted real rank-k update C = sum_e wt_e x_e x_e^T of M rows of D doubles
+// (device or host memory); wt_e = w[e] (directWeights) or 1/w[e] or 1.
+// Returns the full symmetric D x D matrix and the sum of the weights.
+int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
+             const double* h_x, const double* h_w, std::vector<double>& full, double* wtot)
 {
 	RPWA_HIP_CHECK(hipSetDevice(device));
 	IntegralPlan p;
-	int st = integral_plan(device, W, p);
+	int st = integral_plan(device, D, p);
 	if (st != RPWA_OK)
 		return st;
 	hipStream_t s;
@@ -326,8 +330,8 @@ static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W
 	INT_CHECK(hipMalloc(&d_part, partN * sizeof(double)));
 	INT_CHECK(hipMalloc(&d_wsum, p.nWg * sizeof(double)));
 	INT_CHECK(hipMalloc(&d_red, (size_t)p.nTiles * 256 * sizeof(double)));
-	IntKernel k = pick_int_kernel(p.tilesPerPass);
-	// every pass re-reads the amplitudes for its own set of output tiles
+	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk);
+	// every pass re-reads the rows for its own set of output tiles
 	auto launch_passes = [&](IntArgs aa) -> hipError_t {
 		for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
 			aa.tileBase = tb;
@@ -341,15 +345,15 @@ static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W
 	};
 	IntArgs a;
 	a.D = p.D; a.Dp = p.Dp; a.ldsStride = p.ldsStride; a.nT = p.nT; a.nTiles = p.nTiles;
-	a.partials = d_part; a.wsum = d_wsum;
-	if (d_amps) {
-		a.x = d_amps; a.w = d_w; a.M = M; a.accumulate = 0;
+	a.partials = d_part; a.wsum = d_wsum; a.directWeights = directWeights ? 1 : 0;
+	if (d_x) {
+		a.x = d_x; a.w = d_w; a.M = M; a.accumulate = 0;
 		INT_CHECK(launch_passes(a));
 	} else {
 		// stream the host array through two device buffers
-		const uint64_t chunkEv = std::max<uint64_t>(1, (256ull << 20) / (16ull * W));
+		const uint64_t chunkEv = std::max<uint64_t>(1, (256ull << 20) / (8ull * D));
 		for (int b = 0; b < 2; ++b) {
-			INT_CHECK(hipMalloc(&d_stage[b], chunkEv * 16ull * W));
+			INT_CHECK(hipMalloc(&d_stage[b], chunkEv * 8ull * D));
 			if (weighted)
 				INT_CHECK(hipMalloc(&d_wstage[b], chunkEv * sizeof(double)));
 		}
@@ -362,11 +366,9 @@ static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W
 			const uint64_t n = std::min<uint64_t>(chunkEv, M - off);
 			if (!first)
 				INT_CHECK(hipEventSynchronize(done[b]));
-			INT_CHECK(hipMemcpyAsync(d_stage[b], h_amps + 2 * off * W, n * 16ull * W,
-			                         hipMemcpyHostToDevice, s));
+			INT_CHECK(hipMemcpyAsync(d_stage[b], h_x + off * D, n * 8ull * D, hipMemcpyHostToDevice, s));
 			if (weighted)
-				INT_CHECK(hipMemcpyAsync(d_wstage[b], h_w + off, n * sizeof(double),
-				                         hipMemcpyHostToDevice, s));
+				INT_CHECK(hipMemcpyAsync(d_wstage[b], h_w + off, n * sizeof(double), hipMemcpyHostToDevice, s));
 			a.x = d_stage[b]; a.w = weighted ? d_wstage[b] : nullptr; a.M = n;
 			a.accumulate = first ? 0 : 1;
 			INT_CHECK(launch_passes(a));
@@ -378,25 +380,50 @@ static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W
 		hipEventDestroy(done[1]);
 	}
 	const int nb = (int)((p.nTiles * 256 + kColElems - 1) / kColElems);
-	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, s, d_part,
-	                   p.nWg, p.nTiles * 256, d_red);
+	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, s, d_part, p.nWg,
+	                   p.nTiles * 256, d_red);
 	INT_CHECK(hipGetLastError());
 	INT_CHECK(hipMemcpyAsync(C.data(), d_red, C.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-	INT_CHECK(hipMemcpyAsync(wsums.data(), d_wsum, wsums.size() * sizeof(double),
-	                         hipMemcpyDeviceToHost, s));
+	INT_CHECK(hipMemcpyAsync(wsums.data(), d_wsum, wsums.size() * sizeof(double), hipMemcpyDeviceToHost, s));
 	INT_CHECK(hipStreamSynchronize(s));
 #undef INT_CHECK
-	assemble(C, p, W, outSums);
-	if (weighted) {
-		double wtot = 0.;
+	full.assign((size_t)D * D, 0.);
+	for (uint32_t r = 0; r < D; ++r)
+		for (uint32_t c = 0; c < D; ++c)
+			full[(size_t)r * D + c] = tileC(C, p, r, c);
+	if (wtot) {
+		double t = 0.;
 		for (double v : wsums)
-			wtot += v;
+			t += v;
+		*wtot = t;
+	}
+	fail(RPWA_OK);
+	return RPWA_OK;
+}
+
+static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W, const double* d_w,
+                        double* outSums, const double* h_amps, const double* h_w)
+{
+	std::vector<double> C;
+	double wtot = 0.;
+	const uint32_t D = 2 * W;
+	int st = syrk_run(device, d_amps, M, D, d_w, false, h_amps, h_w, C, &wtot);
+	if (st != RPWA_OK)
+		return st;
+	for (uint32_t i = 0; i < W; ++i)
+		for (uint32_t j = 0; j < W; ++j) {
+			const double re = C[(size_t)(2 * i) * D + 2 * j] + C[(size_t)(2 * i + 1) * D + 2 * j + 1];
+			const double im = C[(size_t)(2 * i + 1) * D + 2 * j] - C[(size_t)(2 * i) * D + 2 * j + 1];
+			outSums[2 * ((size_t)i * W + j)] = re;
+			outSums[2 * ((size_t)i * W + j) + 1] = im;
+		}
+	if (d_w || h_w) {
+		// renormalise to the weight integral (ampIntegralMatrix.cc:477-483)
 		const double weightNorm = wtot / (double)M;
 		const double inv = 1 / weightNorm;
 		for (size_t i = 0; i < (size_t)2 * W * W; ++i)
 			outSums[i] *= inv;
 	}
-	fail(RPWA_OK);
 	return RPWA_OK;
 }
 

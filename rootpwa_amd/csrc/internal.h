@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "rootpwa_amd.h"
 
@@ -77,6 +78,9 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s);
+// weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T
+int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
+             const double* h_x, const double* h_w, std::vector<double>& full, double* wtot);
 hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int srcLayout,
                                  const double* d_scale, void* d_dst, hipStream_t s);
 

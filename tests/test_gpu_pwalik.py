@@ -128,3 +128,27 @@ def test_integral_matrix_class(gpu, tmp_path):
     m4.readAscii(f)
     s4, n4 = m4.sums()
     assert n4 == 2000 and np.array_equal(s4, sums) and m4.waveNames() == ["a", "b", "c", "d", "e"]
+
+
+def test_hessian_matches_oracle(gpu, pair):
+    """pwaLikelihood::Hessian (pwaLikelihood.cc:642-859): GPU event sums + host terms."""
+    ds, Lo, Lg = pair
+    par = np.random.default_rng(11).normal(size=Lo.nPars)
+    Hg = Lg.Hessian(par)
+    Ho = Lo.hessian(par)
+    scale = np.max(np.abs(Ho))
+    print("hessian max abs diff / max", np.max(np.abs(Hg - Ho)) / scale)
+    assert np.max(np.abs(Hg - Ho)) <= 1e-10 * scale
+    # covariance = inverse of the Hessian on the free parameters
+    C = Lg.CovarianceMatrix(par)
+    free = [i for i in range(Lo.nPars) if not Lg.parameter(i)[2]]
+    Hf = Ho[np.ix_(free, free)]
+    assert np.allclose(C[np.ix_(free, free)] @ Hf, np.eye(len(free)), atol=1e-6)
+
+
+def test_hessian_mixed_precision(gpu):
+    ds, Lo, Lg = build_pair((3, 9), 2, seed=13, precision=1, nData=1500)
+    par = np.random.default_rng(12).normal(size=Lo.nPars)
+    Hg = Lg.Hessian(par)
+    Ho = Lo.hessian(par)
+    assert np.max(np.abs(Hg - Ho)) <= 1e-5 * np.max(np.abs(Ho))
