@@ -97,6 +97,55 @@ int rpwa_pwalik_gpu_context(const rpwa_pwalik* L, rpwa_lik_ctx** ctx);
 /* number of calls and accumulated total time [s] of FdF, Gradient, DoEval,
  * DoDerivative, Hessian (functionCallEnum order) */
 int rpwa_pwalik_call_info(const rpwa_pwalik* L, uint32_t calls[5], double totalTime[5]);
+/* HessianEigenVectors (partialWaveFit/pwaLikelihood.h:184, .cc:864-911) of a
+ * P x P Hessian: nFree = P - nmbParsFixed eigenvalues in descending order and
+ * the matching eigenvectors as rows [nFree][P] (zero at fixed parameters) */
+int rpwa_pwalik_hessian_eigen(const rpwa_pwalik* L, const double* hessian, double* eigenValues,
+                              double* eigenVectors);
+
+/* ---- fit drivers (highLevelInterface/pwaNloptFit.cc:84-402, pwaFit.cc:64-433) ----
+ * A fit runs the reference's flow on the likelihood: start values from the
+ * seed, minimisation (L-BFGS on FdF, or DoEval + Gradient in Minuit's call
+ * pattern), CorrectParamSigns with the bitwise DoEval check, Hessian
+ * eigenvalues and covariance.  opts == NULL takes the reference's settings:
+ * NLopt flow memory 10, maxEvaluations 50000, xtolRel 1e-4, ftolAbs 1e-6;
+ * Minuit flow maxEvaluations 40000, edmTolerance 1e-10. */
+typedef struct {
+	uint32_t memory;          /* L-BFGS correction pairs */
+	uint32_t maxEvaluations;
+	double xtolRel;           /* 0: off */
+	double ftolAbs;           /* 0: off */
+	double edmTolerance;      /* Minuit flow: stop when EDM < 0.002 * edmTolerance */
+} rpwa_fit_options;
+
+typedef struct {
+	double logLikelihood;     /* the minimum of -lnL the minimiser returned (bitwise DoEval of the result) */
+	double edm;               /* Minuit flow: estimated distance to the minimum */
+	int32_t status;           /* NLopt result codes: >0 success (1 success, 3 ftol, 4 xtol, 5 maxeval,
+	                             7 EDM goal, 8 machine precision), <0 failure (-4 roundoff limited) */
+	int32_t converged;        /* status > 0 and (if computed) a positive-definite Hessian */
+	int32_t hasHessian;       /* the covariance is filled */
+	uint32_t nmbEvaluations;  /* FdF calls (NLopt flow) or DoEval calls (Minuit flow) */
+	uint32_t nmbIterations;
+	uint32_t nmbNonPositiveEigenvalues;
+} rpwa_fit_summary;
+
+/* the start values both drivers draw from seed (pwaNloptFit.cc:183-212,
+ * pwaFit.cc:235-257): for each parameter Uniform(0.01, sqrt(N)) and a sign
+ * flip if a second draw exceeds 0.5, from MT19937 seeded by init_genrand(seed)
+ * (the generator of ROOT's TRandom3); fixed parameters are 0.  out: [P] */
+int rpwa_fit_start_values(const rpwa_pwalik* L, uint32_t seed, double* out);
+/* pwaNloptFit: outPar [P] (after CorrectParamSigns); outCovariance [P][P] or
+ * NULL (saveSpace); outEigenValues [P - nmbParsFixed] or NULL (filled when
+ * checkHessian or a covariance is requested).  A changed likelihood after the
+ * sign flip is an error (RPWA_ERR_INVALID), as the reference throws there. */
+int rpwa_fit_nlopt(const rpwa_pwalik* L, uint32_t seed, int checkHessian, const rpwa_fit_options* opts,
+                   double* outPar, double* outCovariance, double* outEigenValues, rpwa_fit_summary* out);
+/* pwaFit: startPar [P] or NULL (a zero for a free parameter draws a random
+ * start value, pwaFit.cc:250-257); otherwise as rpwa_fit_nlopt */
+int rpwa_fit_minuit(const rpwa_pwalik* L, uint32_t seed, const double* startPar, int checkHessian,
+                    const rpwa_fit_options* opts, double* outPar, double* outCovariance, double* outEigenValues,
+                    rpwa_fit_summary* out);
 
 #ifdef __cplusplus
 }

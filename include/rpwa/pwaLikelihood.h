@@ -123,6 +123,8 @@ public:
 	denseMatrix Hessian(const double* par) const;
 	denseMatrix CovarianceMatrix(const double* par) const;
 	denseMatrix CovarianceMatrix(const denseMatrix& hessian) const;
+	// (eigenvector over all P parameters, eigenvalue), descending eigenvalues
+	std::vector<std::pair<std::vector<double>, double> > HessianEigenVectors(const denseMatrix& hessian) const;
 	std::vector<double> CorrectParamSigns(const double* par) const;
 
 	unsigned int nmbEvents() const { return (unsigned int)_nmbEvents; }

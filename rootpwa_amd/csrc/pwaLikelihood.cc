@@ -916,6 +916,86 @@ denseMatrix pwaLikelihood<complexT>::CovarianceMatrix(const double* par) const
 	return CovarianceMatrix(Hessian(par));
 }
 
+// HessianEigenVectors (pwaLikelihood.cc:864-911): eigen-decomposition of the
+// Hessian reduced to the free parameters; eigenvectors are expanded back to
+// all parameters with zeros at the fixed ones.  Order: descending eigenvalues.
+// The decomposition is the cyclic Jacobi method (Golub & Van Loan, Matrix
+// Computations, sec. 8.5): exact rotations until the off-diagonal norm is
+// below 1e-15 of the Frobenius norm.
+template <typename complexT>
+std::vector<std::pair<std::vector<double>, double> >
+pwaLikelihood<complexT>::HessianEigenVectors(const denseMatrix& hessian) const
+{
+	std::vector<unsigned int> freeIdx;
+	for (unsigned int i = 0; i < _nmbPars; ++i)
+		if (not _parameters[i].fixed())
+			freeIdx.push_back(i);
+	const unsigned int n = (unsigned int)freeIdx.size();
+	std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.);
+	double frob = 0.;
+	for (unsigned int i = 0; i < n; ++i) {
+		v[(size_t)i * n + i] = 1.;
+		for (unsigned int j = 0; j < n; ++j) {
+			// symmetrise: the reference's Hessian is symmetric by construction
+			const double h = 0.5 * (hessian(freeIdx[i], freeIdx[j]) + hessian(freeIdx[j], freeIdx[i]));
+			a[(size_t)i * n + j] = h;
+			frob += h * h;
+		}
+	}
+	const double target = 1e-30 * frob;
+	for (unsigned int sweep = 0; sweep < 100; ++sweep) {
+		double off = 0.;
+		for (unsigned int p = 0; p < n; ++p)
+			for (unsigned int q = p + 1; q < n; ++q)
+				off += 2. * a[(size_t)p * n + q] * a[(size_t)p * n + q];
+		if (off <= target)
+			break;
+		for (unsigned int p = 0; p + 1 < n; ++p)
+			for (unsigned int q = p + 1; q < n; ++q) {
+				const double apq = a[(size_t)p * n + q];
+				if (apq == 0.)
+					continue;
+				// rotation that zeroes a[p][q]: t = tan(theta), the smaller root of
+				// t^2 + 2 tau t - 1 = 0 with tau = (a_qq - a_pp) / (2 a_pq)
+				const double tau = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2. * apq);
+				const double t = (tau >= 0. ? 1. : -1.) / (std::fabs(tau) + std::sqrt(1. + tau * tau));
+				const double c = 1. / std::sqrt(1. + t * t);
+				const double s = t * c;
+				for (unsigned int k = 0; k < n; ++k) {  // columns p, q
+					const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+					a[(size_t)k * n + p] = c * akp - s * akq;
+					a[(size_t)k * n + q] = s * akp + c * akq;
+				}
+				for (unsigned int k = 0; k < n; ++k) {  // rows p, q
+					const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+					a[(size_t)p * n + k] = c * apk - s * aqk;
+					a[(size_t)q * n + k] = s * apk + c * aqk;
+				}
+				a[(size_t)p * n + q] = a[(size_t)q * n + p] = 0.;
+				for (unsigned int k = 0; k < n; ++k) {
+					const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+					v[(size_t)k * n + p] = c * vkp - s * vkq;
+					v[(size_t)k * n + q] = s * vkp + c * vkq;
+				}
+			}
+	}
+	std::vector<unsigned int> order(n);
+	for (unsigned int i = 0; i < n; ++i)
+		order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [&](unsigned int x, unsigned int y) {
+		return a[(size_t)x * n + x] > a[(size_t)y * n + y];
+	});
+	std::vector<std::pair<std::vector<double>, double> > eigen;
+	eigen.reserve(n);
+	for (unsigned int k = 0; k < n; ++k) {
+		std::vector<double> vec(_nmbPars, 0.);
+		for (unsigned int j = 0; j < n; ++j)
+			vec[freeIdx[j]] = v[(size_t)j * n + order[k]];
+		eigen.push_back(std::make_pair(vec, a[(size_t)order[k] * n + order[k]]));
+	}
+	return eigen;
+}
+
 // CorrectParamSigns (pwaLikelihood.cc:986-1010)
 template <typename complexT>
 std::vector<double> pwaLikelihood<complexT>::CorrectParamSigns(const double* par) const

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "rootpwa_amd_pwalik.h"
+#include "rpwa/fitDrivers.h"
 #include "rpwa/pwaLikelihood.h"
 
 namespace rpwa_amd {
@@ -652,4 +653,143 @@ extern "C" int rpwa_pwalik_gpu_context(const rpwa_pwalik* L, rpwa_lik_ctx** ctx)
 	}
 	return RPWA_OK;
 	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_hessian_eigen(const rpwa_pwalik* L, const double* hessian, double* eigenValues,
+                                         double* eigenVectors)
+{
+	GUARD_BEGIN
+	if (!L || !hessian || !eigenValues)
+		return fail("rpwa_pwalik_hessian_eigen", "null pointer");
+	WITH_LIK(L, {
+		const unsigned int P = lik.nmbPars();
+		rpwa::denseMatrix h(P);
+		std::memcpy(h.data.data(), hessian, (size_t)P * P * sizeof(double));
+		const auto eigen = lik.HessianEigenVectors(h);
+		for (size_t k = 0; k < eigen.size(); ++k) {
+			eigenValues[k] = eigen[k].second;
+			if (eigenVectors)
+				std::memcpy(eigenVectors + k * P, eigen[k].first.data(), P * sizeof(double));
+		}
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_fit_start_values(const rpwa_pwalik* L, uint32_t seed, double* out)
+{
+	GUARD_BEGIN
+	if (!L || !out)
+		return fail("rpwa_fit_start_values", "null pointer");
+	WITH_LIK(L, {
+		const std::vector<double> v = rpwa::hli::randomStartValues(lik, seed);
+		std::memcpy(out, v.data(), v.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+// shared part of the two driver entry points
+static rpwa::minimizerOptions fit_options(const rpwa_fit_options* o, const bool minuit)
+{
+	rpwa::minimizerOptions m = minuit ? rpwa::hli::minuitDefaults() : rpwa::minimizerOptions();
+	if (o) {
+		m.memory = o->memory;
+		m.maxEvaluations = o->maxEvaluations;
+		m.xtolRel = o->xtolRel;
+		m.ftolAbs = o->ftolAbs;
+		m.edmTolerance = o->edmTolerance;
+	}
+	return m;
+}
+
+static void fit_outputs(const rpwa::hli::fitResultData& r, const unsigned int P, double* outPar,
+                        double* outCovariance, double* outEigenValues, rpwa_fit_summary* out)
+{
+	std::memcpy(outPar, r.fitParameters.data(), (size_t)P * sizeof(double));
+	if (outCovariance) {
+		if (r.covMatrix.n == P)
+			std::memcpy(outCovariance, r.covMatrix.data.data(), (size_t)P * P * sizeof(double));
+		else
+			std::memset(outCovariance, 0, (size_t)P * P * sizeof(double));
+	}
+	if (outEigenValues && !r.hessianEigenValues.empty())
+		std::memcpy(outEigenValues, r.hessianEigenValues.data(), r.hessianEigenValues.size() * sizeof(double));
+	uint32_t nonPositive = 0;
+	for (double e : r.hessianEigenValues)
+		if (e <= 0.)
+			++nonPositive;
+	out->logLikelihood = r.logLikelihood;
+	out->edm = r.edm;
+	out->status = r.status;
+	out->converged = r.converged ? 1 : 0;
+	out->hasHessian = r.hasHessian ? 1 : 0;
+	out->nmbEvaluations = r.nmbEvaluations;
+	out->nmbIterations = r.nmbIterations;
+	out->nmbNonPositiveEigenvalues = nonPositive;
+}
+
+extern "C" int rpwa_fit_nlopt(const rpwa_pwalik* L, uint32_t seed, int checkHessian, const rpwa_fit_options* opts,
+                              double* outPar, double* outCovariance, double* outEigenValues, rpwa_fit_summary* out)
+{
+	GUARD_BEGIN
+	if (!L || !outPar || !out)
+		return fail("rpwa_fit_nlopt", "null pointer");
+	if (opts && (opts->memory == 0 || opts->maxEvaluations == 0))
+		return fail("rpwa_fit_nlopt", "memory and maxEvaluations must be positive");
+	const rpwa::minimizerOptions m = fit_options(opts, false);
+	WITH_LIK(L, {
+		const rpwa::hli::fitResultData r =
+			rpwa::hli::pwaNloptFit(lik, seed, checkHessian != 0 || outEigenValues, outCovariance == nullptr, false, m);
+		fit_outputs(r, lik.nmbPars(), outPar, outCovariance, outEigenValues, out);
+	});
+	return RPWA_OK;
+	}
+	catch (const rpwa::hli::signConventionError& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_INVALID;
+	}
+	catch (const std::logic_error& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_STATE;
+	}
+	catch (const std::exception& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_HIP;
+	}
+}
+
+extern "C" int rpwa_fit_minuit(const rpwa_pwalik* L, uint32_t seed, const double* startPar, int checkHessian,
+                               const rpwa_fit_options* opts, double* outPar, double* outCovariance,
+                               double* outEigenValues, rpwa_fit_summary* out)
+{
+	GUARD_BEGIN
+	if (!L || !outPar || !out)
+		return fail("rpwa_fit_minuit", "null pointer");
+	if (opts && (opts->memory == 0 || opts->maxEvaluations == 0))
+		return fail("rpwa_fit_minuit", "memory and maxEvaluations must be positive");
+	const rpwa::minimizerOptions m = fit_options(opts, true);
+	WITH_LIK(L, {
+		std::vector<double> start;
+		if (startPar)
+			start.assign(startPar, startPar + lik.nmbPars());
+		const rpwa::hli::fitResultData r = rpwa::hli::pwaFit(lik, seed, startPar ? &start : nullptr,
+		                                                     checkHessian != 0 || outEigenValues,
+		                                                     outCovariance == nullptr, false, m);
+		fit_outputs(r, lik.nmbPars(), outPar, outCovariance, outEigenValues, out);
+	});
+	return RPWA_OK;
+	}
+	catch (const rpwa::hli::signConventionError& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_INVALID;
+	}
+	catch (const std::logic_error& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_STATE;
+	}
+	catch (const std::exception& ex) {
+		set_error(ex.what());
+		return RPWA_ERR_HIP;
+	}
 }

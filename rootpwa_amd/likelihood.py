@@ -64,7 +64,29 @@ PROTOTYPES = {
     "rpwa_pwalik_integral_matrices": (_i, [_vp, _i, _dp, _dp, _dp]),
     "rpwa_pwalik_call_info": (_i, [_vp, _u32p, _dp]),
     "rpwa_pwalik_gpu_context": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "rpwa_pwalik_hessian_eigen": (_i, [_vp, _dp, _dp, _dp]),
+    "rpwa_fit_start_values": (_i, [_vp, _u32, _dp]),
+    "rpwa_fit_nlopt": (_i, [_vp, _u32, _i, _vp, _dp, _dp, _dp, _vp]),
+    "rpwa_fit_minuit": (_i, [_vp, _u32, _dp, _i, _vp, _dp, _dp, _dp, _vp]),
 }
+
+
+class FitOptions(ctypes.Structure):
+    """rpwa_fit_options"""
+    _fields_ = [("memory", ctypes.c_uint32), ("maxEvaluations", ctypes.c_uint32), ("xtolRel", ctypes.c_double),
+                ("ftolAbs", ctypes.c_double), ("edmTolerance", ctypes.c_double)]
+
+
+class FitSummary(ctypes.Structure):
+    """rpwa_fit_summary"""
+    _fields_ = [("logLikelihood", ctypes.c_double), ("edm", ctypes.c_double), ("status", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("hasHessian", ctypes.c_int32), ("nmbEvaluations", ctypes.c_uint32),
+                ("nmbIterations", ctypes.c_uint32), ("nmbNonPositiveEigenvalues", ctypes.c_uint32)]
+
+
+# reference settings of the two drivers (pwaNloptFit.cc:105-107, pwaFit.cc:86-93)
+NLOPT_DEFAULTS = dict(memory=10, maxEvaluations=50000, xtolRel=1e-4, ftolAbs=1e-6, edmTolerance=1e-10)
+MINUIT_DEFAULTS = dict(memory=10, maxEvaluations=40000, xtolRel=0.0, ftolAbs=0.0, edmTolerance=1e-10)
 
 _ready = False
 
@@ -325,6 +347,17 @@ class pwaLikelihood:
         _call("rpwa_pwalik_covariance", self.h, _d(p), _d(C))
         return C
 
+    def HessianEigenVectors(self, hessian):
+        """[(eigenvector over all parameters, eigenvalue)], descending eigenvalues
+        (partialWaveFit/pwaLikelihood.cc:864-911)."""
+        H = np.ascontiguousarray(hessian, dtype=np.float64)
+        P = self.nmbPars()
+        nFree = P - self.nmbParsFixed()
+        vals = np.zeros(max(nFree, 1))
+        vecs = np.zeros((max(nFree, 1), P))
+        _call("rpwa_pwalik_hessian_eigen", self.h, _d(H), _d(vals), _d(vecs))
+        return [(vecs[k].copy(), float(vals[k])) for k in range(nFree)]
+
     def CorrectParamSigns(self, par):
         p = self._par(par)
         out = np.zeros_like(p)
@@ -390,3 +423,104 @@ def initLikelihood(waves, amplitudes, normIntegral, accIntegral=None, rank=1, ma
         L.addAmplitude(w[0], amplitudes[w[0]])
     L.finishInit()
     return L
+
+
+class fitResult:
+    """What fitResult::fill receives in the drivers (pwaNloptFit.cc:385-398), as arrays."""
+
+    def __init__(self, L, par, cov, eig, summary):
+        self.fitParameters = par
+        self.covMatrix = cov
+        self.hessianEigenValues = eig
+        self._logLikelihood = summary.logLikelihood
+        self._converged = bool(summary.converged)
+        self._hasHessian = bool(summary.hasHessian)
+        self.status = summary.status
+        self.edm = summary.edm
+        self.nmbEvaluations = summary.nmbEvaluations
+        self.nmbIterations = summary.nmbIterations
+        self.nmbNonPositiveEigenvalues = summary.nmbNonPositiveEigenvalues
+        self.parNames = [L.parameter(i)[0] for i in range(L.nmbPars())]
+
+    def logLikelihood(self):
+        return self._logLikelihood
+
+    def converged(self):
+        return self._converged
+
+    def covMatrixValid(self):
+        return self._hasHessian
+
+    def fitParameter(self, name):
+        return float(self.fitParameters[self.parNames.index(name)])
+
+
+def _fit(entry, L, seed, checkHessian, saveSpace, opts, defaults, startValues=None):
+    o = dict(defaults)
+    o.update(opts)
+    options = FitOptions(**o)
+    P = L.nmbPars()
+    par = np.zeros(P)
+    cov = None if saveSpace else np.zeros((P, P))
+    wantEig = checkHessian or not saveSpace
+    eig = np.zeros(max(P - L.nmbParsFixed(), 1)) if wantEig else None
+    summary = FitSummary()
+    args = [L.h, int(seed) & 0xFFFFFFFF]
+    if entry == "rpwa_fit_minuit":
+        sv = None if startValues is None else np.ascontiguousarray(startValues, dtype=np.float64)
+        args.append(_d(sv) if sv is not None else None)
+    args += [int(bool(checkHessian)), ctypes.byref(options), _d(par), _d(cov) if cov is not None else None,
+             _d(eig) if eig is not None else None, ctypes.byref(summary)]
+    _call(entry, *args)
+    if eig is not None:
+        eig = eig[:P - L.nmbParsFixed()]
+    return fitResult(L, par, cov, eig, summary)
+
+
+def startValues(likelihood, seed=0):
+    """The drivers' random start values for all parameters (fixed ones 0)."""
+    out = np.zeros(likelihood.nmbPars())
+    _call("rpwa_fit_start_values", likelihood.h, int(seed) & 0xFFFFFFFF, _d(out))
+    return out
+
+
+def pwaNloptFit(likelihood, seed=0, checkHessian=False, saveSpace=False, **opts):
+    """pwaNloptFit (highLevelInterface/pwaNloptFit.cc:84-402) on the GPU likelihood.
+    opts override NLOPT_DEFAULTS."""
+    return _fit("rpwa_fit_nlopt", likelihood, seed, checkHessian, saveSpace, opts, NLOPT_DEFAULTS)
+
+
+def pwaFit(likelihood, seed=0, startValues=None, checkHessian=False, saveSpace=False, **opts):
+    """pwaFit (highLevelInterface/pwaFit.cc:64-433): Minuit's call pattern
+    (DoEval, then Gradient); startValues replaces the start-value file."""
+    return _fit("rpwa_fit_minuit", likelihood, seed, checkHessian, saveSpace, opts, MINUIT_DEFAULTS, startValues)
+
+
+def fitAttempts(likelihood, fitFunction=pwaNloptFit, seed=0, attempts=1, **kwargs):
+    """The attempts loop of pyRootPwa's _pwaFit (pyInterface/module/_fit.py:132-165):
+    seeds drawn with random.randint(1000, 2**32-1) from random.seed(seed), all
+    seeds 0 if seed is 0; returns (results, index of best, index of best converged)."""
+    import random
+    if attempts == 1:
+        seeds = [seed]
+    elif seed == 0:
+        seeds = [0] * attempts
+    else:
+        random.seed(seed)
+        seeds = []
+        for _ in range(attempts):
+            while True:
+                v = random.randint(1000, 2 ** 32 - 1)
+                if v not in seeds:
+                    break
+            seeds.append(v)
+    results, iBest, iBestValid = [], -1, -1
+    best, bestValid = np.inf, np.inf
+    for s in seeds:
+        r = fitFunction(likelihood, seed=s, **kwargs)
+        results.append(r)
+        if r.logLikelihood() < best:
+            best, iBest = r.logLikelihood(), len(results) - 1
+        if r.converged() and r.logLikelihood() < bestValid:
+            bestValid, iBestValid = r.logLikelihood(), len(results) - 1
+    return results, iBest, iBestValid
