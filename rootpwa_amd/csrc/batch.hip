@@ -1,0 +1,1497 @@
+// batch.hip -- batched fits over many bins and starts (include/rootpwa_amd_batch.h).
+//
+// One evaluation of all fits is four kernels, captured once in a hipGraph and
+// replayed for every L-BFGS iteration:
+//   batch_event_kernel   persistent; each workgroup streams a contiguous range
+//                        of 32-event tiles (ranges may span bins) through LDS
+//                        and, for the 64 starts of the tile's bin, computes
+//                        S = A T (FP32 MFMA), sigma, ln sigma (FP64) and
+//                        G += A^H (S / sigma) (FP32 MFMA), writing one partial
+//                        per (workgroup, bin segment);
+//   batch_reduce_kernel  per fit: partials in fixed order (FP64), norm term
+//                        T^H I T and its derivative (FP64), -lnL and gradient;
+//   batch_lbfgs_kernel   per fit: Armijo line search, L-BFGS two-loop
+//                        direction, stopping tests, next trial point;
+//   batch_status_kernel  per bin: does any start still run.
+// Reductions run in a fixed order, so results do not depend on timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "rootpwa_amd_batch.h"
+#include "rpwa/fitDrivers.h"
+
+namespace rpwa_amd {
+namespace {
+
+constexpr int kCols = 64;      // starts per bin in one workgroup: 4 waves x 16 starts
+constexpr int kEv = 32;        // events per tile (MFMA 32x32 rows)
+constexpr int kThreads = 256;  // 4 waves
+constexpr int kMaxMem = 16;    // L-BFGS correction pairs
+constexpr int kPrefetch = 6;   // float4 per thread per tile: 32 * 192 / 4 / 256
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+// global-address-space view, so tile loads are global (not flat) loads and do
+// not share the LDS wait counter
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4 gfloat4;
+
+struct BatchEventArgs {
+	const float* const* binAmps;  // per bin: [tiles * 32][rowLen] float (complex64 pairs)
+	const uint32_t* tileStart;    // [nBins + 1]
+	const uint32_t* nEvents;      // [nBins]
+	const int32_t* binActive;     // [nBins]
+	const float* T32;             // [nBins][64][rowLen]
+	const double* flat2;          // [nBins][64]
+	const uint32_t* wgSlotStart;  // [nWg]
+	float* partX;                 // [nSlots][32 NT][128]
+	double* partLn;               // [nSlots][64]
+	double* partInv;              // [nSlots][64]
+	uint32_t nTiles;
+	uint32_t rowLen;              // 2 Wp, Wp = waves padded to a multiple of 4
+};
+
+__device__ __forceinline__ float swap_pair(float v)
+{
+	// lanes 2k <-> 2k+1 (DPP quad_perm [1,0,3,2])
+	return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ double swap_pair_d(double v)
+{
+	const uint64_t u = (uint64_t)__double_as_longlong(v);
+	const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0xB1, 0xF, 0xF, true);
+	const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0xB1, 0xF, 0xF, true);
+	return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <int KQ>
+__global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventArgs a)
+{
+	constexpr int NT = (KQ + 3) / 4;   // tiles of 32 wave components
+	constexpr int XROWS = 32 * NT;     // padded wave-component rows of the gradient partial
+	constexpr int STRIDE = XROWS + 4;  // LDS row stride [floats]: rows 16 bytes apart in the banks
+	__shared__ __attribute__((aligned(16))) float lds[2 * kEv * STRIDE];
+
+	const int tid = threadIdx.x;
+	const int wave = tid >> 6, lane = tid & 63;
+	const int h = lane >> 5, j = lane & 31;
+	const int cout = j & 1;                    // this lane's column: Re (0) or Im (1) of S
+	const int start = 16 * wave + (j >> 1);    // this lane's fit start
+	constexpr uint32_t rowLen = 8 * KQ;        // 2 Wp floats per event
+	constexpr uint32_t kSteps = 4 * KQ;        // MFMA K steps of S = A T
+	constexpr int nF4 = kEv * (int)rowLen / 4; // float4 per tile
+
+	// padding columns [rowLen, STRIDE) are never written by tile loads
+	for (int idx = tid; idx < 2 * kEv * STRIDE; idx += kThreads)
+		if ((uint32_t)(idx % STRIDE) >= rowLen)
+			lds[idx] = 0.f;
+	__syncthreads();
+
+	const uint32_t G = gridDim.x, g = blockIdx.x;
+	const uint32_t t0 = (uint32_t)(((uint64_t)g * a.nTiles) / G);
+	const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * a.nTiles) / G);
+	if (t0 >= t1)
+		return;
+	uint32_t b = 0;
+	while (a.tileStart[b + 1] <= t0)
+		++b;
+	uint32_t slot = a.wgSlotStart[g];
+
+	for (uint32_t t = t0; t < t1; ++slot, ++b) {
+		const uint32_t segEnd = min(t1, a.tileStart[b + 1]);
+		if (!a.binActive[b]) {
+			t = segEnd;
+			continue;
+		}
+		const float* amps = a.binAmps[b];
+		const uint32_t tileBase = a.tileStart[b];
+		const uint32_t nEv = a.nEvents[b];
+
+		// B operand of S = A T in registers: K step k, lane half h -> wave
+		// component wc = 8 (k / 4) + 4 h + k % 4 (matches the float4 LDS reads)
+		float treg[kSteps];
+		{
+			const float* Tb = a.T32 + ((size_t)b * kCols + start) * rowLen;
+#pragma unroll
+			for (int k = 0; k < (int)kSteps; ++k) {
+				const int wc = 8 * (k >> 2) + 4 * h + (k & 3);
+				const int cin = wc & 1;
+				const float tre = Tb[wc - cin], tim = Tb[wc - cin + 1];
+				// [Sre Sim] = [Are Aim] [[Tre Tim] [-Tim Tre]]
+				treg[k] = (cin == cout) ? tre : (cin == 0 ? tim : -tim);
+			}
+		}
+		const double f2 = a.flat2[(size_t)b * kCols + start];
+
+		f32x16 X[NT];
+#pragma unroll
+		for (int x = 0; x < NT; ++x)
+			X[x] = f32x16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+		double mant = 1.;  // running product of sigma, renormalised per tile
+		int expo = 0;
+		double invAcc = 0.;
+
+		f32x4 pf[kPrefetch];
+#define RPWA_BATCH_GLOAD(tile)                                                                              \
+	do {                                                                                                    \
+		const gfloat4* src_ = (const gfloat4*)(amps + (size_t)((tile) - tileBase) * kEv * rowLen);     \
+		_Pragma("unroll") for (int r_ = 0; r_ < kPrefetch; ++r_)                                            \
+		{                                                                                                   \
+			const int idx_ = min(tid + kThreads * r_, nF4 - 1);                                             \
+			pf[r_] = src_[idx_];                                                                            \
+		}                                                                                                   \
+	} while (0)
+#define RPWA_BATCH_LSTORE(buf)                                                                              \
+	do {                                                                                                    \
+		_Pragma("unroll") for (int r_ = 0; r_ < kPrefetch; ++r_)                                            \
+		{                                                                                                   \
+			const int idx_ = tid + kThreads * r_;                                                           \
+			if (idx_ < nF4) {                                                                               \
+				const int e_ = (4 * idx_) / (int)rowLen, c_ = (4 * idx_) % (int)rowLen;                     \
+				*reinterpret_cast<f32x4*>(&lds[(buf) * kEv * STRIDE + e_ * STRIDE + c_]) = pf[r_];         \
+			}                                                                                               \
+		}                                                                                                   \
+	} while (0)
+
+		RPWA_BATCH_GLOAD(t);
+		RPWA_BATCH_LSTORE(0);
+		__syncthreads();
+		int buf = 0;
+		for (; t < segEnd; ++t) {
+			const bool more = t + 1 < segEnd;
+			if (more)
+				RPWA_BATCH_GLOAD(t + 1);
+			const float* L = lds + buf * kEv * STRIDE;
+
+			// S[32 events][32 columns of this wave] = A_tile T
+			f32x16 S = f32x16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+			for (int q = 0; q < KQ; ++q) {
+				const float4 av = *reinterpret_cast<const float4*>(&L[j * STRIDE + 8 * q + 4 * h]);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, treg[4 * q + 0], S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, treg[4 * q + 1], S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, treg[4 * q + 2], S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, treg[4 * q + 3], S, 0, 0, 0);
+			}
+
+			// sigma = |S|^2 + flat^2 (FP64), ln sigma via the running product,
+			// S <- S / sigma for the gradient; register v holds event
+			// 8 (v / 4) + 4 h + v % 4 of the tile, column j
+			const uint32_t evBase = (t - tileBase) * kEv;
+#pragma unroll
+			for (int v = 0; v < 16; ++v) {
+				const int row = 8 * (v >> 2) + 4 * h + (v & 3);
+				const bool valid = evBase + (uint32_t)row < nEv;
+				const float me = S[v];
+				const float other = swap_pair(me);
+				const float re = cout ? other : me, im = cout ? me : other;
+				double sig = fma((double)re, (double)re, fma((double)im, (double)im, f2));
+				sig = valid ? sig : 1.;
+				const float inv = __builtin_amdgcn_rcpf((float)sig);
+				S[v] = valid ? me * inv : 0.f;
+				if ((v & 1) == cout) {  // the pair splits the event terms
+					mant *= sig;
+					invAcc += valid ? (double)inv : 0.;
+				}
+			}
+			{
+				// mant = m 2^e with m in [1, 2): keep m, carry e (sigma > 0 is finite or inf/nan)
+				const uint64_t u = (uint64_t)__double_as_longlong(mant);
+				const int e = (int)((u >> 52) & 0x7FF);
+				if (e != 0 && e != 0x7FF) {
+					expo += e - 1023;
+					mant = __longlong_as_double((long long)((u & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+				}
+			}
+
+			// X[wave components][columns] += A_tile^T (S / sigma): K step p pairs
+			// register p of both lane halves (events 8 (p / 4) + 4 h + p % 4)
+#pragma unroll
+			for (int p = 0; p < 16; ++p) {
+				const int row = 8 * (p >> 2) + 4 * h + (p & 3);
+#pragma unroll
+				for (int x = 0; x < NT; ++x)
+					X[x] = __builtin_amdgcn_mfma_f32_32x32x2f32(L[row * STRIDE + 32 * x + j], S[p], X[x], 0, 0, 0);
+			}
+			if (more)
+				RPWA_BATCH_LSTORE(buf ^ 1);
+			__syncthreads();
+			buf ^= 1;
+		}
+
+		// partial of this (workgroup, bin segment)
+		float* px = a.partX + (size_t)slot * XROWS * 128;
+#pragma unroll
+		for (int x = 0; x < NT; ++x)
+#pragma unroll
+			for (int v = 0; v < 16; ++v) {
+				const int wc = 32 * x + 8 * (v >> 2) + 4 * h + (v & 3);
+				px[(size_t)wc * 128 + 32 * wave + j] = X[x][v];
+			}
+		double lnv = log(mant) + (double)expo * 0.69314718055994530942;
+		lnv += swap_pair_d(lnv);
+		lnv += __shfl_xor(lnv, 32);
+		double inv = invAcc + swap_pair_d(invAcc);
+		inv += __shfl_xor(inv, 32);
+		if (h == 0 && cout == 0) {
+			a.partLn[(size_t)slot * kCols + start] = lnv;
+			a.partInv[(size_t)slot * kCols + start] = inv;
+		}
+	}
+}
+
+#undef RPWA_BATCH_GLOAD
+#undef RPWA_BATCH_LSTORE
+
+typedef void (*BatchEventKernel)(const BatchEventArgs);
+
+// KQ = Wp / 4, Wp = waves padded to a multiple of 8
+BatchEventKernel pick_event_kernel(int KQ)
+{
+	switch (KQ) {
+	case 2: return batch_event_kernel<2>;
+	case 4: return batch_event_kernel<4>;
+	case 6: return batch_event_kernel<6>;
+	case 8: return batch_event_kernel<8>;
+	case 10: return batch_event_kernel<10>;
+	case 12: return batch_event_kernel<12>;
+	case 14: return batch_event_kernel<14>;
+	case 16: return batch_event_kernel<16>;
+	case 18: return batch_event_kernel<18>;
+	case 20: return batch_event_kernel<20>;
+	case 22: return batch_event_kernel<22>;
+	default: return batch_event_kernel<24>;
+	}
+}
+
+// minimiser state of all fits (FP64), fit f = bin * 64 + start
+struct FitState {
+	double* x;      // [nFits][P] accepted point
+	double* g;      // gradient at x
+	double* d;      // search direction
+	double* xt;     // trial point (evaluated next)
+	double* gt;     // gradient at xt (from the reduction)
+	double* S;      // [nFits][m][P]
+	double* Y;
+	double* rho;    // [nFits][m]
+	double* f;      // value at x
+	double* ft;     // value at xt
+	double* alpha;
+	double* dphi0;
+	double* gamma;
+	int32_t* status;
+	int32_t* nEval;
+	int32_t* backtracks;
+	int32_t* mCount;
+	int32_t* mHead;
+};
+
+// status codes of a batched fit (> 0 finished, as NLopt's result codes)
+constexpr int ST_INIT = 0;
+constexpr int ST_SEARCH = 1;
+constexpr int ST_FTOL = 3;
+constexpr int ST_XTOL = 4;
+constexpr int ST_MAXEVAL = 5;
+constexpr int ST_NOPROGRESS = -4;
+constexpr int ST_BADSTART = -1;
+constexpr int ST_PAD = -100;  // padding fit (start >= nStarts): never runs
+
+__device__ __forceinline__ bool running(int st) { return st == ST_INIT || st == ST_SEARCH; }
+
+struct BatchReduceArgs {
+	const float* partX;
+	const double* partLn;
+	const double* partInv;
+	const uint32_t* binSlot;  // [nBins + 1]
+	const int32_t* binActive;
+	const float* T32;
+	const double* acc;        // [nBins][W][W] complex, normalised
+	const double* totAcc;     // [nBins]
+	const double* xt;
+	double* ft;
+	double* gt;
+	uint32_t W, P, rowLen, nStarts, XROWS;
+};
+
+// per fit: gradient partials in slot order, norm term, value and gradient
+__global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs r)
+{
+	const uint32_t b = blockIdx.x / kCols, s = blockIdx.x % kCols;
+	if (!r.binActive[b] || s >= r.nStarts)
+		return;
+	const size_t fit = blockIdx.x;
+	const uint32_t i = threadIdx.x;
+	__shared__ double red[128];
+	const float* Tf = r.T32 + fit * r.rowLen;
+	double normPart = 0.;
+	if (i < r.W) {
+		double gre = 0., gim = 0.;
+		for (uint32_t sl = r.binSlot[b]; sl < r.binSlot[b + 1]; ++sl) {
+			const float* X = r.partX + (size_t)sl * r.XROWS * 128;
+			const double x00 = X[(size_t)(2 * i) * 128 + 2 * s], x11 = X[(size_t)(2 * i + 1) * 128 + 2 * s + 1];
+			const double x01 = X[(size_t)(2 * i) * 128 + 2 * s + 1], x10 = X[(size_t)(2 * i + 1) * 128 + 2 * s];
+			gre += x00 + x11;
+			gim += x01 - x10;
+		}
+		// u_i = sum_j T_j conj(I_ij); norm = sum_i Re(T_i conj(u_i)); d norm / d T_i = 2 u_i
+		const double* I = r.acc + ((size_t)b * r.W + i) * r.W * 2;
+		double ure = 0., uim = 0.;
+		for (uint32_t jw = 0; jw < r.W; ++jw) {
+			const double tr = Tf[2 * jw], ti = Tf[2 * jw + 1];
+			const double ir = I[2 * jw], ii = I[2 * jw + 1];
+			ure += tr * ir + ti * ii;
+			uim += ti * ir - tr * ii;
+		}
+		const double tir = Tf[2 * i], tii = Tf[2 * i + 1];
+		normPart = tir * ure + tii * uim;
+		const double dre = -2. * gre + 2. * ure, dim = -2. * gim + 2. * uim;
+		double* gout = r.gt + fit * r.P;
+		if (i == 0)
+			gout[0] = dre;
+		else {
+			gout[2 * i - 1] = dre;
+			gout[2 * i] = dim;
+		}
+	}
+	red[i] = normPart;
+	__syncthreads();
+	for (uint32_t w = 64; w > 0; w >>= 1) {
+		if (i < w)
+			red[i] += red[i + w];
+		__syncthreads();
+	}
+	if (i == 0) {
+		double lnSum = 0., invSum = 0.;
+		for (uint32_t sl = r.binSlot[b]; sl < r.binSlot[b + 1]; ++sl) {
+			lnSum += r.partLn[(size_t)sl * kCols + s];
+			invSum += r.partInv[(size_t)sl * kCols + s];
+		}
+		const double flat = r.xt[fit * r.P + r.P - 1];
+		const double eta = r.totAcc[b];
+		r.ft[fit] = -lnSum + (red[0] + flat * flat * eta);
+		r.gt[fit * r.P + r.P - 1] = -2. * flat * invSum + 2. * flat * eta;
+	}
+}
+
+// trial point -> FP32 production amplitudes and flat^2 for the event kernel
+__device__ void write_trial(const double* xt, uint32_t W, uint32_t P, float* T32, double* flat2)
+{
+	for (uint32_t i = threadIdx.x; i < W; i += blockDim.x) {
+		const double re = i == 0 ? xt[0] : xt[2 * i - 1];
+		const double im = i == 0 ? 0. : xt[2 * i];
+		T32[2 * i] = (float)re;
+		T32[2 * i + 1] = (float)im;
+	}
+	if (threadIdx.x == 0) {
+		const double f = xt[P - 1];
+		*flat2 = f * f;
+	}
+}
+
+struct BatchPrepArgs {
+	const double* xt;
+	float* T32;
+	double* flat2;
+	uint32_t W, P, rowLen, nStarts;
+};
+
+// evaluation at given points (no minimiser step)
+__global__ __launch_bounds__(64) void batch_prepare_kernel(const BatchPrepArgs p)
+{
+	const uint32_t fit = blockIdx.x, s = fit % kCols;
+	if (s >= p.nStarts)
+		return;
+	write_trial(p.xt + (size_t)fit * p.P, p.W, p.P, p.T32 + (size_t)fit * p.rowLen, p.flat2 + fit);
+}
+
+struct BatchLbfgsArgs {
+	FitState st;
+	float* T32;
+	double* flat2;
+	uint32_t W, P, rowLen, m, maxEval;
+	double xtolRel, ftolAbs;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+	for (int k = 32; k >= 1; k >>= 1)
+		v += __shfl_xor(v, k);
+	return v;
+}
+
+// one wave per fit; each lane holds parameters lane, lane + 64, lane + 128
+__global__ __launch_bounds__(64) void batch_lbfgs_kernel(const BatchLbfgsArgs a)
+{
+	const size_t fit = blockIdx.x;
+	const FitState& st = a.st;
+	const int status = st.status[fit];
+	if (!running(status))
+		return;
+	const int lane = threadIdx.x;
+	const uint32_t P = a.P, m = a.m;
+	double* x = st.x + fit * P;
+	double* g = st.g + fit * P;
+	double* d = st.d + fit * P;
+	double* xt = st.xt + fit * P;
+	const double* gt = st.gt + fit * P;
+	double* Sm = st.S + fit * m * P;
+	double* Ym = st.Y + fit * m * P;
+	double* rho = st.rho + fit * m;
+	constexpr int kPer = 3;  // P <= 192
+	const double ft = st.ft[fit];
+	int nEval = st.nEval[fit] + 1;
+	int newStatus = status;
+	double alpha = st.alpha[fit];
+	bool newDirection = false;
+
+	if (status == ST_INIT) {
+		if (!isfinite(ft)) {
+			newStatus = ST_BADSTART;
+		} else {
+			for (int q = 0; q < kPer; ++q) {
+				const uint32_t k = lane + 64 * q;
+				if (k < P) {
+					x[k] = xt[k];
+					g[k] = gt[k];
+				}
+			}
+			if (lane == 0)
+				st.f[fit] = ft;
+			newStatus = ST_SEARCH;
+			newDirection = true;
+		}
+	} else {
+		const double f = st.f[fit];
+		const double dphi0 = st.dphi0[fit];
+		if (isfinite(ft) && ft <= f + 1e-4 * alpha * dphi0) {
+			// accept the trial point
+			bool xConv = a.xtolRel > 0.;
+			double sy = 0., ss = 0., yy = 0.;
+			const int head = st.mHead[fit];
+			double* sNew = Sm + (size_t)head * P;
+			double* yNew = Ym + (size_t)head * P;
+			for (int q = 0; q < kPer; ++q) {
+				const uint32_t k = lane + 64 * q;
+				if (k < P) {
+					const double sk = xt[k] - x[k], yk = gt[k] - g[k];
+					if (!(fabs(sk) < a.xtolRel * 0.5 * (fabs(xt[k]) + fabs(x[k])) || xt[k] == x[k]))
+						xConv = false;
+					sNew[k] = sk;
+					yNew[k] = yk;
+					sy += sk * yk;
+					ss += sk * sk;
+					yy += yk * yk;
+					x[k] = xt[k];
+					g[k] = gt[k];
+				}
+			}
+			sy = wave_sum(sy);
+			ss = wave_sum(ss);
+			yy = wave_sum(yy);
+			xConv = __all(xConv);
+			if (lane == 0)
+				st.f[fit] = ft;
+			st.backtracks[fit] = 0;
+			if (a.ftolAbs > 0. && fabs(ft - f) < a.ftolAbs)
+				newStatus = ST_FTOL;
+			else if (xConv)
+				newStatus = ST_XTOL;
+			else {
+				if (sy > 1e-12 * sqrt(ss * yy)) {
+					if (lane == 0) {
+						rho[head] = 1. / sy;
+						st.gamma[fit] = sy / yy;
+						st.mHead[fit] = (head + 1) % (int)m;
+						st.mCount[fit] = min(st.mCount[fit] + 1, (int)m);
+					}
+				}
+				newDirection = true;
+			}
+		} else {
+			const int bt = st.backtracks[fit] + 1;
+			if (bt > 60) {
+				newStatus = ST_NOPROGRESS;
+			} else {
+				double an = isfinite(ft) ? -dphi0 * alpha * alpha / (2. * (ft - f - dphi0 * alpha)) : 0.5 * alpha;
+				if (!isfinite(an) || an < 0.1 * alpha || an > 0.5 * alpha)
+					an = (an < 0.1 * alpha || !isfinite(an)) ? 0.1 * alpha : 0.5 * alpha;
+				alpha = an;
+				if (lane == 0)
+					st.backtracks[fit] = bt;
+				if (alpha == 0.)
+					newStatus = ST_NOPROGRESS;
+			}
+		}
+	}
+	__syncthreads();  // history scalars written by lane 0 above
+
+	if (newDirection) {
+		// d = -H g (two-loop recursion, newest pair first)
+		const int count = st.mCount[fit], head = st.mHead[fit];
+		const double gamma = st.gamma[fit];
+		double q[kPer];
+		for (int r = 0; r < kPer; ++r) {
+			const uint32_t k = lane + 64 * r;
+			q[r] = k < P ? g[k] : 0.;
+		}
+		double al[kMaxMem];
+#pragma unroll
+		for (int c = 0; c < kMaxMem; ++c) {
+			if (c < count) {
+				const int slot = (head - 1 - c + 2 * (int)m) % (int)m;
+				const double* sv = Sm + (size_t)slot * P;
+				const double* yv = Ym + (size_t)slot * P;
+				double s = 0.;
+				for (int r = 0; r < kPer; ++r) {
+					const uint32_t k = lane + 64 * r;
+					if (k < P)
+						s += sv[k] * q[r];
+				}
+				al[c] = rho[slot] * wave_sum(s);
+				for (int r = 0; r < kPer; ++r) {
+					const uint32_t k = lane + 64 * r;
+					if (k < P)
+						q[r] -= al[c] * yv[k];
+				}
+			}
+		}
+		if (count > 0)
+			for (int r = 0; r < kPer; ++r)
+				q[r] *= gamma;
+#pragma unroll
+		for (int c = kMaxMem - 1; c >= 0; --c) {
+			if (c < count) {
+				const int slot = (head - 1 - c + 2 * (int)m) % (int)m;
+				const double* sv = Sm + (size_t)slot * P;
+				const double* yv = Ym + (size_t)slot * P;
+				double s = 0.;
+				for (int r = 0; r < kPer; ++r) {
+					const uint32_t k = lane + 64 * r;
+					if (k < P)
+						s += yv[k] * q[r];
+				}
+				const double beta = rho[slot] * wave_sum(s);
+				for (int r = 0; r < kPer; ++r) {
+					const uint32_t k = lane + 64 * r;
+					if (k < P)
+						q[r] += sv[k] * (al[c] - beta);
+				}
+			}
+		}
+		double gd = 0., gg = 0.;
+		for (int r = 0; r < kPer; ++r) {
+			const uint32_t k = lane + 64 * r;
+			if (k < P) {
+				gd -= g[k] * q[r];
+				gg += g[k] * g[k];
+			}
+		}
+		gd = wave_sum(gd);
+		gg = wave_sum(gg);
+		if (count == 0 || !(gd < 0.)) {
+			// steepest descent with a step of unit length; history restarts
+			for (int r = 0; r < kPer; ++r) {
+				const uint32_t k = lane + 64 * r;
+				if (k < P)
+					d[k] = -g[k];
+			}
+			if (lane == 0 && count > 0) {
+				st.mCount[fit] = 0;
+				st.mHead[fit] = 0;
+			}
+			gd = -gg;
+			alpha = gg > 0. ? 1. / sqrt(gg) : 0.;
+		} else {
+			for (int r = 0; r < kPer; ++r) {
+				const uint32_t k = lane + 64 * r;
+				if (k < P)
+					d[k] = -q[r];
+			}
+			alpha = 1.;
+		}
+		if (lane == 0)
+			st.dphi0[fit] = gd;
+		if (gg == 0.)
+			newStatus = ST_FTOL;  // stationary point
+	}
+	if (newStatus == ST_SEARCH && nEval >= (int)a.maxEval)
+		newStatus = ST_MAXEVAL;
+	if (lane == 0) {
+		st.status[fit] = newStatus;
+		st.nEval[fit] = nEval;
+		st.alpha[fit] = alpha;
+	}
+	if (newStatus == ST_SEARCH) {
+		for (int r = 0; r < kPer; ++r) {
+			const uint32_t k = lane + 64 * r;
+			if (k < P)
+				xt[k] = x[k] + alpha * d[k];
+		}
+		__syncthreads();
+		write_trial(xt, a.W, P, a.T32 + fit * a.rowLen, a.flat2 + fit);
+	}
+}
+
+struct BatchStatusArgs {
+	const int32_t* status;
+	int32_t* binActive;
+	int32_t* nRunning;
+	uint32_t nBins;
+};
+
+// one block: bins with a running start, and the number of running fits
+__global__ __launch_bounds__(256) void batch_status_kernel(const BatchStatusArgs a)
+{
+	__shared__ int32_t red[256];
+	int32_t count = 0;
+	for (uint32_t b = threadIdx.x; b < a.nBins; b += 256) {
+		int32_t n = 0;
+		for (int s = 0; s < kCols; ++s)
+			n += running(a.status[(size_t)b * kCols + s]) ? 1 : 0;
+		a.binActive[b] = n > 0 ? 1 : 0;
+		count += n;
+	}
+	red[threadIdx.x] = count;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w)
+			red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		*a.nRunning = red[0];
+}
+
+struct BatchInitArgs {
+	FitState st;
+	float* T32;
+	double* flat2;
+	uint32_t W, P, rowLen, nStarts;
+};
+
+// fit state at the start values xt (set by the host)
+__global__ __launch_bounds__(64) void batch_init_kernel(const BatchInitArgs a)
+{
+	const size_t fit = blockIdx.x;
+	const uint32_t s = fit % kCols;
+	if (threadIdx.x == 0) {
+		a.st.status[fit] = s < a.nStarts ? ST_INIT : ST_PAD;
+		a.st.nEval[fit] = 0;
+		a.st.backtracks[fit] = 0;
+		a.st.mCount[fit] = 0;
+		a.st.mHead[fit] = 0;
+		a.st.alpha[fit] = 0.;
+		a.st.gamma[fit] = 1.;
+		a.st.f[fit] = 0.;
+	}
+	if (s < a.nStarts)
+		write_trial(a.st.xt + fit * a.P, a.W, a.P, a.T32 + fit * a.rowLen, a.flat2 + fit);
+}
+
+// complex128 [N][W] -> scaled complex64 [tiles * 32][rowLen] (zero padding)
+__global__ void batch_transform_kernel(const double2* __restrict__ src, uint64_t nEvents, uint32_t W,
+                                       uint32_t Wp, uint64_t nRows, const double* __restrict__ scale,
+                                       float2* __restrict__ dst)
+{
+	const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= nRows * Wp)
+		return;
+	const uint64_t e = idx / Wp;
+	const uint32_t w = (uint32_t)(idx % Wp);
+	float2 out = make_float2(0.f, 0.f);
+	if (e < nEvents && w < W) {
+		const double2 v = src[e * W + w];
+		const double f = scale[w];
+		out = make_float2((float)(v.x * f), (float)(v.y * f));
+	}
+	dst[idx] = out;
+}
+
+uint64_t splitmix64(uint64_t x)
+{
+	x += 0x9E3779B97F4A7C15ull;
+	x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+	x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+	return x ^ (x >> 31);
+}
+
+}  // namespace
+}  // namespace rpwa_amd
+
+using namespace rpwa_amd;
+
+struct rpwa_batch {
+	int device = 0;
+	uint32_t W = 0, Wp = 0, rowLen = 0, NT = 0, nStarts = 0, P = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	int nCu = 0;
+	struct Bin {
+		uint64_t nEvents = 0;
+		uint32_t nTiles = 0;
+		float* d_amps = nullptr;
+		std::vector<double> acc;  // [W][W] complex, normalised
+		double totAcc = 1.;
+	};
+	std::vector<Bin> bins;
+	bool finished = false;
+	uint32_t nTiles = 0, nSlots = 0, nWg = 0, wgPerCu = 0, wgPerCuFixed = 0, maxWgPerCu = 1;
+	uint64_t totalEvents = 0;
+	uint64_t lastEvaluations = 0;
+	// device
+	float** d_binAmps = nullptr;
+	uint32_t* d_tileStart = nullptr;
+	uint32_t* d_nEvents = nullptr;
+	int32_t* d_binActive = nullptr;
+	uint32_t* d_wgSlotStart = nullptr;
+	uint32_t* d_binSlot = nullptr;
+	float* d_T32 = nullptr;
+	double* d_flat2 = nullptr;
+	float* d_partX = nullptr;
+	double* d_partLn = nullptr;
+	double* d_partInv = nullptr;
+	double* d_acc = nullptr;
+	double* d_totAcc = nullptr;
+	int32_t* d_nRunning = nullptr;
+	FitState st{};
+	uint32_t memory = 0;  // history depth the state was allocated for
+	std::vector<void*> allocations;
+	hipGraphExec_t graph = nullptr;
+	uint32_t graphMem = 0, graphMaxEval = 0;
+	double graphXtol = -1., graphFtol = -1.;
+};
+
+namespace {
+
+int check_batch(const rpwa_batch* b, const char* fn, bool mustBeFinished)
+{
+	if (!b) {
+		set_error(std::string(fn) + ": null batch");
+		return RPWA_ERR_INVALID;
+	}
+	if (mustBeFinished && !b->finished) {
+		set_error(std::string(fn) + ": rpwa_batch_finish has not been called");
+		return RPWA_ERR_STATE;
+	}
+	return RPWA_OK;
+}
+
+template <typename T>
+int dalloc(rpwa_batch* b, T** p, size_t n)
+{
+	void* q = nullptr;
+	RPWA_HIP_CHECK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+	b->allocations.push_back(q);
+	*p = reinterpret_cast<T*>(q);
+	return RPWA_OK;
+}
+
+#define RPWA_TRY(expr)              \
+	do {                            \
+		const int rc_ = (expr);     \
+		if (rc_ != RPWA_OK)         \
+			return rc_;             \
+	} while (0)
+
+size_t nFits(const rpwa_batch* b) { return b->bins.size() * kCols; }
+
+// static plan: workgroup g streams tiles [g T / G, (g + 1) T / G); a slot per
+// (workgroup, bin segment); each bin's slots are contiguous and in workgroup order
+void make_plan(const rpwa_batch* b, uint32_t nWg, std::vector<uint32_t>& wgSlotStart, std::vector<uint32_t>& binSlot,
+               uint32_t& nSlots)
+{
+	const uint32_t nBins = (uint32_t)b->bins.size();
+	std::vector<uint32_t> tileStart(nBins + 1, 0);
+	for (uint32_t i = 0; i < nBins; ++i)
+		tileStart[i + 1] = tileStart[i] + b->bins[i].nTiles;
+	const uint32_t T = tileStart[nBins];
+	wgSlotStart.assign(nWg, 0);
+	std::vector<std::vector<uint32_t> > slotsOfBin(nBins);
+	uint32_t slot = 0;
+	for (uint32_t g = 0; g < nWg; ++g) {
+		const uint32_t t0 = (uint32_t)(((uint64_t)g * T) / nWg), t1 = (uint32_t)(((uint64_t)(g + 1) * T) / nWg);
+		wgSlotStart[g] = slot;
+		if (t0 >= t1)
+			continue;
+		uint32_t bi = 0;
+		while (tileStart[bi + 1] <= t0)
+			++bi;
+		for (uint32_t t = t0; t < t1; ++bi) {
+			slotsOfBin[bi].push_back(slot++);
+			t = std::min(t1, tileStart[bi + 1]);
+		}
+	}
+	nSlots = slot;
+	binSlot.assign(nBins + 1, 0);
+	for (uint32_t i = 0; i < nBins; ++i) {
+		// contiguous by construction: workgroups are visited in order
+		binSlot[i] = slotsOfBin[i].front();
+		binSlot[i + 1] = slotsOfBin[i].back() + 1;
+	}
+}
+
+int upload_plan(rpwa_batch* b, uint32_t nWg)
+{
+	std::vector<uint32_t> wgSlotStart, binSlot;
+	uint32_t nSlots;
+	make_plan(b, nWg, wgSlotStart, binSlot, nSlots);
+	if (nSlots > b->nSlots) {
+		set_error("rpwa_batch: internal error, partial slots exceed the allocation");
+		return RPWA_ERR_STATE;
+	}
+	RPWA_HIP_CHECK(hipMemcpy(b->d_wgSlotStart, wgSlotStart.data(), nWg * sizeof(uint32_t), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_binSlot, binSlot.data(), binSlot.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+	b->nWg = nWg;
+	return RPWA_OK;
+}
+
+BatchEventArgs event_args(const rpwa_batch* b)
+{
+	BatchEventArgs a;
+	a.binAmps = b->d_binAmps;
+	a.tileStart = b->d_tileStart;
+	a.nEvents = b->d_nEvents;
+	a.binActive = b->d_binActive;
+	a.T32 = b->d_T32;
+	a.flat2 = b->d_flat2;
+	a.wgSlotStart = b->d_wgSlotStart;
+	a.partX = b->d_partX;
+	a.partLn = b->d_partLn;
+	a.partInv = b->d_partInv;
+	a.nTiles = b->nTiles;
+	a.rowLen = b->rowLen;
+	return a;
+}
+
+BatchReduceArgs reduce_args(const rpwa_batch* b)
+{
+	BatchReduceArgs r;
+	r.partX = b->d_partX;
+	r.partLn = b->d_partLn;
+	r.partInv = b->d_partInv;
+	r.binSlot = b->d_binSlot;
+	r.binActive = b->d_binActive;
+	r.T32 = b->d_T32;
+	r.acc = b->d_acc;
+	r.totAcc = b->d_totAcc;
+	r.xt = b->st.xt;
+	r.ft = b->st.ft;
+	r.gt = b->st.gt;
+	r.W = b->W;
+	r.P = b->P;
+	r.rowLen = b->rowLen;
+	r.nStarts = b->nStarts;
+	r.XROWS = 32 * b->NT;
+	return r;
+}
+
+int launch_event(rpwa_batch* b)
+{
+	hipLaunchKernelGGL(pick_event_kernel((int)(b->Wp / 4)), dim3(b->nWg), dim3(kThreads), 0, b->stream, event_args(b));
+	RPWA_HIP_CHECK(hipGetLastError());
+	return RPWA_OK;
+}
+
+int launch_reduce(rpwa_batch* b)
+{
+	hipLaunchKernelGGL(batch_reduce_kernel, dim3((uint32_t)nFits(b)), dim3(128), 0, b->stream, reduce_args(b));
+	RPWA_HIP_CHECK(hipGetLastError());
+	return RPWA_OK;
+}
+
+int launch_status(rpwa_batch* b)
+{
+	BatchStatusArgs s;
+	s.status = b->st.status;
+	s.binActive = b->d_binActive;
+	s.nRunning = b->d_nRunning;
+	s.nBins = (uint32_t)b->bins.size();
+	hipLaunchKernelGGL(batch_status_kernel, dim3(1), dim3(256), 0, b->stream, s);
+	RPWA_HIP_CHECK(hipGetLastError());
+	return RPWA_OK;
+}
+
+int launch_lbfgs(rpwa_batch* b, uint32_t maxEval, double xtolRel, double ftolAbs)
+{
+	BatchLbfgsArgs a;
+	a.st = b->st;
+	a.T32 = b->d_T32;
+	a.flat2 = b->d_flat2;
+	a.W = b->W;
+	a.P = b->P;
+	a.rowLen = b->rowLen;
+	a.m = b->memory;
+	a.maxEval = maxEval;
+	a.xtolRel = xtolRel;
+	a.ftolAbs = ftolAbs;
+	hipLaunchKernelGGL(batch_lbfgs_kernel, dim3((uint32_t)nFits(b)), dim3(64), 0, b->stream, a);
+	RPWA_HIP_CHECK(hipGetLastError());
+	return RPWA_OK;
+}
+
+int alloc_state(rpwa_batch* b, uint32_t memory)
+{
+	if (b->memory == memory && b->st.x)
+		return RPWA_OK;
+	const size_t F = nFits(b), P = b->P;
+	FitState& s = b->st;
+	RPWA_TRY(dalloc(b, &s.x, F * P));
+	RPWA_TRY(dalloc(b, &s.g, F * P));
+	RPWA_TRY(dalloc(b, &s.d, F * P));
+	if (!s.xt) {
+		RPWA_TRY(dalloc(b, &s.xt, F * P));
+		RPWA_TRY(dalloc(b, &s.gt, F * P));
+		RPWA_TRY(dalloc(b, &s.ft, F));
+	}
+	RPWA_TRY(dalloc(b, &s.S, F * memory * P));
+	RPWA_TRY(dalloc(b, &s.Y, F * memory * P));
+	RPWA_TRY(dalloc(b, &s.rho, F * memory));
+	RPWA_TRY(dalloc(b, &s.f, F));
+	RPWA_TRY(dalloc(b, &s.alpha, F));
+	RPWA_TRY(dalloc(b, &s.dphi0, F));
+	RPWA_TRY(dalloc(b, &s.gamma, F));
+	RPWA_TRY(dalloc(b, &s.status, F));
+	RPWA_TRY(dalloc(b, &s.nEval, F));
+	RPWA_TRY(dalloc(b, &s.backtracks, F));
+	RPWA_TRY(dalloc(b, &s.mCount, F));
+	RPWA_TRY(dalloc(b, &s.mHead, F));
+	b->memory = memory;
+	if (b->graph) {
+		(void)hipGraphExecDestroy(b->graph);
+		b->graph = nullptr;
+	}
+	return RPWA_OK;
+}
+
+// padded [nBins * 64][P] from the caller's [nBins][nStarts][P]
+void pad_params(const rpwa_batch* b, const double* par, std::vector<double>& out)
+{
+	const size_t P = b->P;
+	out.assign(nFits(b) * P, 0.);
+	for (size_t bi = 0; bi < b->bins.size(); ++bi)
+		for (uint32_t s = 0; s < b->nStarts; ++s)
+			std::memcpy(&out[(bi * kCols + s) * P], par + (bi * b->nStarts + s) * P, P * sizeof(double));
+}
+
+int time_event(rpwa_batch* b, uint32_t reps, double* ms)
+{
+	RPWA_HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+	for (uint32_t r = 0; r < reps; ++r)
+		RPWA_TRY(launch_event(b));
+	RPWA_HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+	RPWA_HIP_CHECK(hipEventSynchronize(b->ev1));
+	float t = 0.f;
+	RPWA_HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+	*ms = (double)t / std::max<uint32_t>(reps, 1);
+	return RPWA_OK;
+}
+
+void host_start_values(const rpwa_batch* b, uint32_t seed, std::vector<double>& out)
+{
+	const uint32_t P = b->P;
+	out.assign(b->bins.size() * b->nStarts * (size_t)P, 0.);
+	for (size_t bi = 0; bi < b->bins.size(); ++bi) {
+		const double sqrtN = std::sqrt((double)b->bins[bi].nEvents);
+		for (uint32_t s = 0; s < b->nStarts; ++s) {
+			const uint32_t fitSeed =
+				(uint32_t)splitmix64(((uint64_t)seed << 32) + ((uint64_t)bi << 16) + (uint64_t)s);
+			rpwa::mt19937Uniform random(fitSeed);
+			double* p = &out[(bi * b->nStarts + s) * (size_t)P];
+			for (uint32_t k = 0; k < P; ++k) {
+				double v = random.uniform(0.01, sqrtN);
+				if (random.rndm() > 0.5)
+					v = -v;
+				p[k] = v;
+			}
+		}
+	}
+}
+
+}  // namespace
+
+extern "C" int rpwa_batch_create(rpwa_batch** out, int device, uint32_t nmbWaves, uint32_t nmbStarts)
+{
+	clear_error();
+	if (!out) {
+		set_error("rpwa_batch_create: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	*out = nullptr;
+	if (nmbWaves == 0 || nmbWaves > RPWA_BATCH_MAX_WAVES || nmbStarts == 0 || nmbStarts > RPWA_BATCH_MAX_STARTS) {
+		set_error("rpwa_batch_create: need 1 <= nmbWaves <= 96 and 1 <= nmbStarts <= 64");
+		return RPWA_ERR_INVALID;
+	}
+	int nDev = 0;
+	if (hipGetDeviceCount(&nDev) != hipSuccess || device < 0 || device >= nDev) {
+		set_error("rpwa_batch_create: no GPU with index " + std::to_string(device));
+		return RPWA_ERR_INVALID;
+	}
+	std::unique_ptr<rpwa_batch> b(new rpwa_batch);
+	b->device = device;
+	b->W = nmbWaves;
+	b->Wp = (nmbWaves + 7) / 8 * 8;
+	b->rowLen = 2 * b->Wp;
+	b->NT = (b->rowLen + 31) / 32;
+	b->nStarts = nmbStarts;
+	b->P = 2 * nmbWaves;
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	RPWA_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+	RPWA_HIP_CHECK(hipEventCreate(&b->ev0));
+	RPWA_HIP_CHECK(hipEventCreate(&b->ev1));
+	RPWA_HIP_CHECK(hipDeviceGetAttribute(&b->nCu, hipDeviceAttributeMultiprocessorCount, device));
+	*out = b.release();
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_destroy(rpwa_batch* b)
+{
+	clear_error();
+	if (!b) {
+		set_error("rpwa_batch_destroy: null batch");
+		return RPWA_ERR_INVALID;
+	}
+	(void)hipSetDevice(b->device);
+	if (b->stream)
+		(void)hipStreamSynchronize(b->stream);
+	if (b->graph)
+		(void)hipGraphExecDestroy(b->graph);
+	for (auto& bin : b->bins)
+		if (bin.d_amps)
+			(void)hipFree(bin.d_amps);
+	for (void* p : b->allocations)
+		(void)hipFree(p);
+	if (b->ev0)
+		(void)hipEventDestroy(b->ev0);
+	if (b->ev1)
+		(void)hipEventDestroy(b->ev1);
+	if (b->stream)
+		(void)hipStreamDestroy(b->stream);
+	delete b;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_add_bin(rpwa_batch* b, const double* amps, uint64_t nmbEvents, int onDevice,
+                                  const double* normSums, uint64_t nmbNormEvents, const double* accSums,
+                                  uint64_t nmbAccEvents, uint32_t* binIndex)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_add_bin", false));
+	if (b->finished) {
+		set_error("rpwa_batch_add_bin: rpwa_batch_finish was called already");
+		return RPWA_ERR_STATE;
+	}
+	if (!amps || !normSums || nmbEvents == 0 || nmbNormEvents == 0 || (accSums && nmbAccEvents == 0)) {
+		set_error("rpwa_batch_add_bin: null pointer or empty input");
+		return RPWA_ERR_INVALID;
+	}
+	const uint32_t W = b->W;
+	// pwaLikelihood::addNormIntegral / addAccIntegral (pwaLikelihood.cc:1068-1107):
+	// element = sums / nmbEvents; eta = accEvents / normEvents
+	std::vector<double> nrm(2 * (size_t)W * W), acc(2 * (size_t)W * W);
+	const double* as = accSums ? accSums : normSums;
+	const double nAcc = (double)nmbNormEvents;  // accMatrix.setNmbEvents(_numbAccEvents)
+	for (size_t k = 0; k < (size_t)W * W; ++k) {
+		nrm[2 * k] = normSums[2 * k] / (double)nmbNormEvents;
+		nrm[2 * k + 1] = normSums[2 * k + 1] / (double)nmbNormEvents;
+		acc[2 * k] = as[2 * k] / nAcc;
+		acc[2 * k + 1] = as[2 * k + 1] / nAcc;
+	}
+	rpwa_batch::Bin bin;
+	bin.nEvents = nmbEvents;
+	bin.nTiles = (uint32_t)((nmbEvents + kEv - 1) / kEv);
+	bin.totAcc = accSums ? (double)nmbAccEvents / (double)nmbNormEvents : 1.;
+	// finishInit with normalised amplitudes (pwaLikelihood.cc:1311-1334)
+	std::vector<double> scale(W);
+	for (uint32_t i = 0; i < W; ++i) {
+		const double ni = nrm[2 * ((size_t)i * W + i)];
+		if (!(ni > 0.)) {
+			set_error("rpwa_batch_add_bin: wave " + std::to_string(i) +
+			          " has a zero normalisation integral (not supported in batched fits)");
+			return RPWA_ERR_INVALID;
+		}
+		scale[i] = 1. / std::sqrt(ni);
+	}
+	bin.acc.resize(2 * (size_t)W * W);
+	for (uint32_t i = 0; i < W; ++i)
+		for (uint32_t j = 0; j < W; ++j) {
+			const double d = std::sqrt(nrm[2 * ((size_t)i * W + i)]) * std::sqrt(nrm[2 * ((size_t)j * W + j)]);
+			bin.acc[2 * ((size_t)i * W + j)] = acc[2 * ((size_t)i * W + j)] / d;
+			bin.acc[2 * ((size_t)i * W + j) + 1] = acc[2 * ((size_t)i * W + j) + 1] / d;
+		}
+	RPWA_HIP_CHECK(hipSetDevice(b->device));
+	const uint64_t nRows = (uint64_t)bin.nTiles * kEv;
+	RPWA_HIP_CHECK(hipMalloc(&bin.d_amps, nRows * b->rowLen * sizeof(float)));
+	double* d_scale = nullptr;
+	double2* d_src = nullptr;
+	int rc = RPWA_OK;
+	do {
+		if (hipMalloc(&d_scale, W * sizeof(double)) != hipSuccess ||
+		    hipMemcpy(d_scale, scale.data(), W * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+			rc = RPWA_ERR_HIP;
+			break;
+		}
+		const double2* src = reinterpret_cast<const double2*>(amps);
+		if (!onDevice) {
+			if (hipMalloc(&d_src, nmbEvents * W * sizeof(double2)) != hipSuccess ||
+			    hipMemcpy(d_src, amps, nmbEvents * W * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+				rc = RPWA_ERR_NOMEM;
+				break;
+			}
+			src = d_src;
+		}
+		const uint64_t n = nRows * b->Wp;
+		hipLaunchKernelGGL(batch_transform_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, b->stream, src,
+		                   nmbEvents, W, b->Wp, nRows, d_scale, reinterpret_cast<float2*>(bin.d_amps));
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+			rc = RPWA_ERR_HIP;
+	} while (false);
+	if (d_scale)
+		(void)hipFree(d_scale);
+	if (d_src)
+		(void)hipFree(d_src);
+	if (rc != RPWA_OK) {
+		(void)hipFree(bin.d_amps);
+		set_error("rpwa_batch_add_bin: device allocation, copy or transform failed");
+		return rc;
+	}
+	if (binIndex)
+		*binIndex = (uint32_t)b->bins.size();
+	b->totalEvents += nmbEvents;
+	b->bins.push_back(std::move(bin));
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_finish(rpwa_batch* b)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_finish", false));
+	if (b->finished)
+		return RPWA_OK;
+	if (b->bins.empty()) {
+		set_error("rpwa_batch_finish: no bins");
+		return RPWA_ERR_STATE;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(b->device));
+	const uint32_t nBins = (uint32_t)b->bins.size();
+	std::vector<uint32_t> tileStart(nBins + 1, 0), nEv(nBins);
+	std::vector<float*> ampPtr(nBins);
+	std::vector<double> acc, totAcc(nBins);
+	for (uint32_t i = 0; i < nBins; ++i) {
+		tileStart[i + 1] = tileStart[i] + b->bins[i].nTiles;
+		nEv[i] = (uint32_t)b->bins[i].nEvents;
+		ampPtr[i] = b->bins[i].d_amps;
+		acc.insert(acc.end(), b->bins[i].acc.begin(), b->bins[i].acc.end());
+		totAcc[i] = b->bins[i].totAcc;
+	}
+	b->nTiles = tileStart[nBins];
+	int occ = 0;
+	RPWA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_event_kernel((int)(b->Wp / 4)), kThreads, 0));
+	b->maxWgPerCu = (uint32_t)std::max(1, std::min(occ, 4));
+	b->nSlots = b->nCu * b->maxWgPerCu + nBins;
+	const size_t F = nFits(b);
+	RPWA_TRY(dalloc(b, &b->d_binAmps, nBins));
+	RPWA_TRY(dalloc(b, &b->d_tileStart, nBins + 1));
+	RPWA_TRY(dalloc(b, &b->d_nEvents, nBins));
+	RPWA_TRY(dalloc(b, &b->d_binActive, nBins));
+	RPWA_TRY(dalloc(b, &b->d_wgSlotStart, (size_t)b->nCu * b->maxWgPerCu));
+	RPWA_TRY(dalloc(b, &b->d_binSlot, nBins + 1));
+	RPWA_TRY(dalloc(b, &b->d_T32, F * b->rowLen));
+	RPWA_TRY(dalloc(b, &b->d_flat2, F));
+	RPWA_TRY(dalloc(b, &b->d_partX, (size_t)b->nSlots * 32 * b->NT * 128));
+	RPWA_TRY(dalloc(b, &b->d_partLn, (size_t)b->nSlots * kCols));
+	RPWA_TRY(dalloc(b, &b->d_partInv, (size_t)b->nSlots * kCols));
+	RPWA_TRY(dalloc(b, &b->d_acc, acc.size()));
+	RPWA_TRY(dalloc(b, &b->d_totAcc, nBins));
+	RPWA_TRY(dalloc(b, &b->d_nRunning, 1));
+	RPWA_TRY(dalloc(b, &b->st.xt, F * b->P));
+	RPWA_TRY(dalloc(b, &b->st.gt, F * b->P));
+	RPWA_TRY(dalloc(b, &b->st.ft, F));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_binAmps, ampPtr.data(), nBins * sizeof(float*), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_tileStart, tileStart.data(), (nBins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_nEvents, nEv.data(), nBins * sizeof(uint32_t), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_acc, acc.data(), acc.size() * sizeof(double), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_totAcc, totAcc.data(), nBins * sizeof(double), hipMemcpyHostToDevice));
+	// T = 0 (padding waves stay zero), flat^2 = 1 for every slot
+	RPWA_HIP_CHECK(hipMemset(b->d_T32, 0, F * b->rowLen * sizeof(float)));
+	std::vector<double> ones(F, 1.);
+	RPWA_HIP_CHECK(hipMemcpy(b->d_flat2, ones.data(), F * sizeof(double), hipMemcpyHostToDevice));
+	std::vector<int32_t> active(nBins, 1);
+	RPWA_HIP_CHECK(hipMemcpy(b->d_binActive, active.data(), nBins * sizeof(int32_t), hipMemcpyHostToDevice));
+	b->finished = true;
+
+	// autotuner: event-kernel time at the drivers' start values for each
+	// number of workgroups per CU the kernel's occupancy allows
+	std::vector<double> start;
+	host_start_values(b, 1u, start);
+	std::vector<double> padded;
+	pad_params(b, start.data(), padded);
+	RPWA_HIP_CHECK(hipMemcpy(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice));
+	BatchPrepArgs pa{b->st.xt, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
+	hipLaunchKernelGGL(batch_prepare_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, pa);
+	RPWA_HIP_CHECK(hipGetLastError());
+	uint32_t best = 1;
+	if (b->wgPerCuFixed) {
+		best = std::min(b->wgPerCuFixed, b->maxWgPerCu);
+	} else if (b->maxWgPerCu > 1) {
+		double bestMs = 1e300;
+		for (uint32_t k = 1; k <= b->maxWgPerCu; ++k) {
+			RPWA_TRY(upload_plan(b, b->nCu * k));
+			RPWA_TRY(launch_event(b));  // warm-up
+			double ms = 0.;
+			RPWA_TRY(time_event(b, 3, &ms));
+			if (std::getenv("RPWA_VERBOSE"))
+				fprintf(stderr, "[rpwa batch autotune] %u WG/CU: %.3f ms\n", k, ms);
+			if (ms < bestMs) {
+				bestMs = ms;
+				best = k;
+			}
+		}
+	}
+	b->wgPerCu = best;
+	RPWA_TRY(upload_plan(b, b->nCu * best));
+	RPWA_HIP_CHECK(hipStreamSynchronize(b->stream));
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_nmb_pars(const rpwa_batch* b, uint32_t* nmbPars)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_nmb_pars", false));
+	if (!nmbPars) {
+		set_error("rpwa_batch_nmb_pars: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	*nmbPars = b->P;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_nmb_bins(const rpwa_batch* b, uint32_t* nmbBins)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_nmb_bins", false));
+	if (!nmbBins) {
+		set_error("rpwa_batch_nmb_bins: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	*nmbBins = (uint32_t)b->bins.size();
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_eval(rpwa_batch* b, const double* par, double* value, double* grad)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_eval", true));
+	if (!par || !value) {
+		set_error("rpwa_batch_eval: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(b->device));
+	const size_t F = nFits(b), P = b->P, nBins = b->bins.size();
+	std::vector<double> padded;
+	pad_params(b, par, padded);
+	RPWA_HIP_CHECK(hipMemcpyAsync(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice,
+	                              b->stream));
+	std::vector<int32_t> active(nBins, 1);
+	RPWA_HIP_CHECK(hipMemcpyAsync(b->d_binActive, active.data(), nBins * sizeof(int32_t), hipMemcpyHostToDevice,
+	                              b->stream));
+	BatchPrepArgs pa{b->st.xt, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
+	hipLaunchKernelGGL(batch_prepare_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, pa);
+	RPWA_HIP_CHECK(hipGetLastError());
+	RPWA_TRY(launch_event(b));
+	RPWA_TRY(launch_reduce(b));
+	std::vector<double> ft(F), gt;
+	RPWA_HIP_CHECK(hipMemcpyAsync(ft.data(), b->st.ft, F * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+	if (grad) {
+		gt.resize(F * P);
+		RPWA_HIP_CHECK(hipMemcpyAsync(gt.data(), b->st.gt, F * P * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+	}
+	RPWA_HIP_CHECK(hipStreamSynchronize(b->stream));
+	for (size_t bi = 0; bi < nBins; ++bi)
+		for (uint32_t s = 0; s < b->nStarts; ++s) {
+			value[bi * b->nStarts + s] = ft[bi * kCols + s];
+			if (grad)
+				std::memcpy(grad + (bi * b->nStarts + s) * P, &gt[(bi * kCols + s) * P], P * sizeof(double));
+		}
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_start_values(const rpwa_batch* b, uint32_t seed, double* out)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_start_values", false));
+	if (!out) {
+		set_error("rpwa_batch_start_values: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	std::vector<double> v;
+	host_start_values(b, seed, v);
+	std::memcpy(out, v.data(), v.size() * sizeof(double));
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_fit(rpwa_batch* b, const double* startPar, uint32_t seed, const rpwa_fit_options* opts,
+                              double* outPar, double* outValue, int32_t* outStatus, uint32_t* outNmbEval)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_fit", true));
+	uint32_t memory = 10, maxEval = 50000;
+	double xtolRel = 1e-4, ftolAbs = 1e-6;
+	if (opts) {
+		memory = opts->memory;
+		maxEval = opts->maxEvaluations;
+		xtolRel = opts->xtolRel;
+		ftolAbs = opts->ftolAbs;
+	}
+	if (memory == 0 || memory > (uint32_t)kMaxMem || maxEval == 0) {
+		set_error("rpwa_batch_fit: need 1 <= memory <= 16 and maxEvaluations > 0");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(b->device));
+	RPWA_TRY(alloc_state(b, memory));
+	const size_t F = nFits(b), P = b->P, nBins = b->bins.size();
+	std::vector<double> start;
+	if (!startPar) {
+		host_start_values(b, seed, start);
+		startPar = start.data();
+	}
+	std::vector<double> padded;
+	pad_params(b, startPar, padded);
+	RPWA_HIP_CHECK(hipMemcpyAsync(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice,
+	                              b->stream));
+	BatchInitArgs ia{b->st, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
+	hipLaunchKernelGGL(batch_init_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, ia);
+	RPWA_HIP_CHECK(hipGetLastError());
+	RPWA_TRY(launch_status(b));
+
+	// one iteration = evaluate every running fit, reduce, minimiser step, bin flags
+	if (!b->graph || b->graphMem != memory || b->graphMaxEval != maxEval || b->graphXtol != xtolRel ||
+	    b->graphFtol != ftolAbs) {
+		if (b->graph) {
+			(void)hipGraphExecDestroy(b->graph);
+			b->graph = nullptr;
+		}
+		hipGraph_t g = nullptr;
+		RPWA_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+		int rc = launch_event(b);
+		if (rc == RPWA_OK)
+			rc = launch_reduce(b);
+		if (rc == RPWA_OK)
+			rc = launch_lbfgs(b, maxEval, xtolRel, ftolAbs);
+		if (rc == RPWA_OK)
+			rc = launch_status(b);
+		const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+		if (rc != RPWA_OK)
+			return rc;
+		RPWA_HIP_CHECK(ec);
+		const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(g);
+		RPWA_HIP_CHECK(ei);
+		b->graphMem = memory;
+		b->graphMaxEval = maxEval;
+		b->graphXtol = xtolRel;
+		b->graphFtol = ftolAbs;
+	}
+	int32_t nRunning = 1;
+	uint64_t iterations = 0;
+	const uint64_t maxIterations = (uint64_t)maxEval + 1;
+	while (nRunning > 0 && iterations < maxIterations) {
+		const uint64_t chunk = std::min<uint64_t>(8, maxIterations - iterations);
+		for (uint64_t k = 0; k < chunk; ++k)
+			RPWA_HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
+		iterations += chunk;
+		RPWA_HIP_CHECK(hipMemcpyAsync(&nRunning, b->d_nRunning, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+		RPWA_HIP_CHECK(hipStreamSynchronize(b->stream));
+	}
+	b->lastEvaluations = iterations;
+
+	std::vector<double> x(F * P), f(F);
+	std::vector<int32_t> status(F), nEval(F);
+	RPWA_HIP_CHECK(hipMemcpy(x.data(), b->st.x, F * P * sizeof(double), hipMemcpyDeviceToHost));
+	RPWA_HIP_CHECK(hipMemcpy(f.data(), b->st.f, F * sizeof(double), hipMemcpyDeviceToHost));
+	RPWA_HIP_CHECK(hipMemcpy(status.data(), b->st.status, F * sizeof(int32_t), hipMemcpyDeviceToHost));
+	RPWA_HIP_CHECK(hipMemcpy(nEval.data(), b->st.nEval, F * sizeof(int32_t), hipMemcpyDeviceToHost));
+	for (size_t bi = 0; bi < nBins; ++bi)
+		for (uint32_t s = 0; s < b->nStarts; ++s) {
+			const size_t fi = bi * kCols + s, o = bi * b->nStarts + s;
+			double* xp = &x[fi * P];
+			if (status[fi] == ST_BADSTART)  // the start point itself
+				std::memcpy(xp, &padded[fi * P], P * sizeof(double));
+			// CorrectParamSigns (pwaLikelihood.cc:986-1010) for rank 1, one sector:
+			// a negative anchor flips the sector; the flat amplitude is |flat|
+			if (xp[0] < 0.)
+				for (size_t k = 0; k + 1 < P; ++k)
+					xp[k] = -xp[k];
+			xp[P - 1] = std::fabs(xp[P - 1]);
+			if (outPar)
+				std::memcpy(outPar + o * P, xp, P * sizeof(double));
+			if (outValue)
+				outValue[o] = status[fi] == ST_BADSTART ? NAN : f[fi];
+			if (outStatus)
+				outStatus[o] = status[fi] == ST_SEARCH || status[fi] == ST_INIT ? ST_MAXEVAL : status[fi];
+			if (outNmbEval)
+				outNmbEval[o] = (uint32_t)nEval[fi];
+		}
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_set_tile(rpwa_batch* b, uint32_t wgPerCu)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_set_tile", false));
+	b->wgPerCuFixed = wgPerCu;
+	if (b->finished) {
+		RPWA_HIP_CHECK(hipSetDevice(b->device));
+		const uint32_t k = wgPerCu ? std::min(wgPerCu, b->maxWgPerCu) : b->wgPerCu;
+		RPWA_TRY(upload_plan(b, b->nCu * k));
+		b->wgPerCu = k;
+		if (b->graph) {
+			(void)hipGraphExecDestroy(b->graph);
+			b->graph = nullptr;
+		}
+	}
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_info(const rpwa_batch* b, uint64_t info[8])
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_info", false));
+	if (!info) {
+		set_error("rpwa_batch_info: null pointer");
+		return RPWA_ERR_INVALID;
+	}
+	info[0] = b->bins.size();
+	info[1] = b->nTiles;
+	info[2] = b->nSlots;
+	info[3] = b->nWg;
+	info[4] = b->wgPerCu;
+	info[5] = b->Wp;
+	info[6] = b->totalEvents;
+	info[7] = b->lastEvaluations;
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_batch_time_event_kernel(rpwa_batch* b, uint32_t reps, double* ms)
+{
+	clear_error();
+	RPWA_TRY(check_batch(b, "rpwa_batch_time_event_kernel", true));
+	if (!ms || reps == 0) {
+		set_error("rpwa_batch_time_event_kernel: null pointer or zero repetitions");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(b->device));
+	std::vector<int32_t> active(b->bins.size(), 1);
+	RPWA_HIP_CHECK(hipMemcpy(b->d_binActive, active.data(), active.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+	return time_event(b, reps, ms);
+}

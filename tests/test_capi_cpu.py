@@ -12,7 +12,7 @@ from oracle import oracle as O
 from tests import helpers as H
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", f) for f in ("rootpwa_amd.h", "rootpwa_amd_pwalik.h")]
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("rootpwa_amd.h", "rootpwa_amd_pwalik.h", "rootpwa_amd_batch.h")]
 
 
 def declared_functions(path):
@@ -31,9 +31,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_bindings_cover_the_headers():
-    from rootpwa_amd import capi, likelihood
+    from rootpwa_amd import batch, capi, likelihood
     names = [n for h in HEADERS for n in declared_functions(h)]
-    bound = set(capi.PROTOTYPES) | set(likelihood.PROTOTYPES)
+    bound = set(capi.PROTOTYPES) | set(likelihood.PROTOTYPES) | set(batch.PROTOTYPES)
     assert [n for n in names if n not in bound] == []
 
 
@@ -224,3 +224,17 @@ def test_fit_drivers_refuse_unfinished_likelihood():
     assert e.value.code == capi.RPWA_ERR_STATE
     with pytest.raises(capi.RpwaError):
         PL.pwaFit(L, seed=1, memory=0)
+
+
+def test_batch_errors_without_gpu():
+    from rootpwa_amd import batch as B
+    from rootpwa_amd import capi
+    with pytest.raises(capi.RpwaError) as e:
+        B.BatchFit(97, 4)  # more waves than the kernel holds
+    assert e.value.code == capi.RPWA_ERR_INVALID
+    with pytest.raises(capi.RpwaError):
+        B.BatchFit(8, 65)  # more starts than one workgroup's columns
+    if capi.device_count() == 0:
+        with pytest.raises(capi.RpwaError) as e:
+            B.BatchFit(8, 4)
+        assert "no GPU" in str(e.value)
