@@ -26,7 +26,9 @@ sys.path.insert(0, ROOT)
 METRIC = ("lnL+grad evals/s, 2M ev×88 waves (mixed vs FP64, |ΔlnL|<1e-3); fits/hr")
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+FP32_MFMA_PEAK_TFS = 157.3
 SEED = 20261202 + 1
+SEED_BATCH = 20261202 + 3
 STREAM_TRUE_T, STREAM_MC, STREAM_DATA, STREAM_START = 2, 3, 4, 5
 
 
@@ -68,6 +70,13 @@ def main():
     ap.add_argument("--oracle-threads", type=int, default=16)
     ap.add_argument("--skip-oracle", action="store_true", help="skip the full-size oracle check (profiling runs)")
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the batched-fit leg")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="profiling runs: only the batched leg, printed as its own JSON line")
+    ap.add_argument("--batch-fits", type=int, default=1,
+                    help="timed batched fit runs (different seeds; the mean time is reported)")
+    ap.add_argument("--batch-bins", type=int, default=48)
+    ap.add_argument("--batch-starts", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -84,6 +93,9 @@ def main():
         sys.exit(1)
     dev = capi.device_info(0)
     log("device", dev)
+    if args.only_batch:
+        print(json.dumps({"batched": batched_leg(args, log)}), flush=True)
+        return
 
     read_gbs, copy_gbs = capi.bandwidth_probe(4 << 30, 10)
     log("bandwidth probe: read %.0f GB/s, copy %.0f GB/s (4 GiB)" % (read_gbs, copy_gbs))
@@ -205,6 +217,8 @@ def main():
                           "evals/s scaled by %d/%d to the full event count" % (args.cpu_evals, S, N, el, S, N))
         log("cpu baseline:", cpu)
 
+    batched = None if args.skip_batch else batched_leg(args, log)
+
     bytes_mix = float(N) * W * 8
     bytes_64 = float(N) * W * 16
     ach_mix = bytes_mix / r_mix["kernel_s"] / 1e9
@@ -244,10 +258,109 @@ def main():
         "bandwidth_probe": {"read_GBs": read_gbs, "copy_GBs": copy_gbs, "bytes": 4 << 30},
         "check": check,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if cpu:
         out["vs_cpu"] = r_mix["rate"] / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(args, log):
+    """Batched fits: 48 bins with data counts log-uniform in [20k, 400k], MC pools
+    of 5x, 88 waves in one sector, rank 1, 64 random starts per bin (3072 fits),
+    complex64 amplitudes, FP32-MFMA contraction; one timed rpwa_batch_fit."""
+    from rootpwa_amd import batch as B
+    from rootpwa_amd import synth
+    W, S = 88, args.batch_starts
+    sizes = synth.bin_sizes(SEED_BATCH, args.batch_bins)
+    bf = B.BatchFit(W, S)
+    t = time.perf_counter()
+    synth.fill_batch(bf, SEED_BATCH, sizes, log=log)
+    gen_s = time.perf_counter() - t
+    bf.finish()
+    ev_ms = bf.timeEventKernel(10)
+    flops = float(sizes.sum()) * W * S * 16
+    ach = flops / (ev_ms * 1e-3) / 1e12
+    log("batched: %d bins, %d events, event kernel %.3f ms = %.1f TFLOP/s algorithmic (%.1f%% of FP32 MFMA peak)"
+        % (len(sizes), int(sizes.sum()), ev_ms, ach, 100 * ach / FP32_MFMA_PEAK_TFS))
+    bf.fit(seed=1)  # untimed: graph instantiation and warm caches
+    runs = max(1, args.batch_fits)
+    t = time.perf_counter()
+    for rep in range(runs):
+        par, val, st, ne = bf.fit(seed=2 + rep)
+    el = (time.perf_counter() - t) / runs
+    info = bf.info()
+    nfits = len(sizes) * S
+    best = val.min(axis=1)
+    within = (val <= best[:, None] + 0.5).sum(axis=1)
+    u, c = np.unique(st, return_counts=True)
+    out = {
+        "fits_per_hr": nfits / el * 3600.0, "wall_s": el, "fits": nfits, "bins": len(sizes), "starts": S,
+        "waves": W, "events": int(sizes.sum()), "iterations": info["lastEvaluations"],
+        "evals_per_fit_mean": float(ne.mean()), "evals_per_fit_max": int(ne.max()),
+        "status": {str(k): int(v) for k, v in zip(u, c)},
+        "starts_within_0.5_of_best": {"mean": float(within.mean()), "min": int(within.min())},
+        "generation_s": gen_s, "seed": SEED_BATCH,
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": ach / FP32_MFMA_PEAK_TFS, "kernel": "batch_event_kernel (persistent, FP32 MFMA)",
+                     "kernel_ms": ev_ms, "flops_per_launch": flops, "workgroups": info["workgroups"]},
+    }
+    log("batched fits: %.3g fits/hr (%.3f s for %d fits, %d iterations, %.1f evaluations per fit)"
+        % (out["fits_per_hr"], el, nfits, info["lastEvaluations"], ne.mean()))
+    if not args.skip_oracle:
+        # the batch's lnL and gradient against the oracle at the best fit of the
+        # smallest and the largest bin (production amplitudes rounded to FP32, as
+        # the batch contracts them; data regenerated from the same counters)
+        from oracle import oracle as O
+        pts = par.copy()
+        pts[..., :-1] = pts[..., :-1].astype(np.float32).astype(np.float64)
+        v, g = bf.eval(pts)
+        checks = []
+        for b in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+            n = int(sizes[b])
+            d_data, sums, M, _ = synth.generate_bin(synth.bin_seed(SEED_BATCH, b), W, n)
+            data = d_data.download((n, W), np.complex128)
+            d_data.free()
+            Lo = O.OracleLikelihood((0, W), 1)
+            Lo.add_integrals(sums, M)
+            Lo.add_amplitudes(data)
+            Lo.finish_init()
+            del data
+            s_best = int(np.argmin(val[b]))
+            f_ref, g_ref = Lo.fdf(pts[b, s_best], threads=args.oracle_threads)
+            checks.append({"bin": b, "events": n, "dlnL": abs(v[b, s_best] - f_ref),
+                           "grad_rel": float(np.max(np.abs(g[b, s_best] - g_ref)) / np.max(np.abs(g_ref)))})
+            del Lo
+        out["check"] = checks
+        log("batched oracle check:", checks)
+    if not args.skip_cpu:
+        # the oracle's single-thread FdF on one bin's likelihood (rank 1, 88 waves), timed
+        # on a sample and scaled to the mean bin size and the mean evaluations per fit
+        from oracle import oracle as O
+        n = 40_000
+        d_data, sums, M, _ = synth.generate_bin(synth.bin_seed(SEED_BATCH, 999), W, n)
+        data = d_data.download((n, W), np.complex128)
+        d_data.free()
+        Lc = O.OracleLikelihood((0, W), 1)
+        Lc.add_integrals(sums, M)
+        Lc.add_amplitudes(data)
+        Lc.finish_init()
+        p = np.random.default_rng(3).normal(size=Lc.nPars) * np.sqrt(n / W)
+        Lc.fdf(p)
+        reps = 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            Lc.fdf(p)
+        s_eval = (time.perf_counter() - t) / reps * (float(sizes.mean()) / n)
+        cpu_fits_hr = 3600.0 / (s_eval * float(ne.mean()))
+        out["cpu_baseline"] = {
+            "value": cpu_fits_hr, "unit": "fits/hr", "cores": 1, "kind": "port",
+            "sample": "%d FdF calls of the single-thread C oracle on a %d-event bin (88 waves, rank 1), "
+                      "scaled to the mean bin size %d and %.1f evaluations per fit" % (reps, n, int(sizes.mean()),
+                                                                                     ne.mean())}
+        out["vs_cpu"] = out["fits_per_hr"] / cpu_fits_hr
+        log("batched cpu baseline: %.3g fits/hr" % cpu_fits_hr)
+    return out
 
 
 if __name__ == "__main__":

@@ -89,7 +89,8 @@ int rpwa_batch_set_tile(rpwa_batch* b, uint32_t wgPerCu);
  * [7] batched evaluations of the last rpwa_batch_fit */
 int rpwa_batch_info(const rpwa_batch* b, uint64_t info[8]);
 /* average duration [ms] of the event kernel over reps launches at the current
- * evaluation point, timed with HIP events on the batch's stream */
+ * evaluation point (after 5 untimed launches), timed with HIP events on the
+ * batch's stream */
 int rpwa_batch_time_event_kernel(rpwa_batch* b, uint32_t reps, double* ms);
 
 #ifdef __cplusplus

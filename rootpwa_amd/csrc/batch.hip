@@ -58,7 +58,7 @@ struct BatchEventArgs {
 	double* partLn;               // [nSlots][64]
 	double* partInv;              // [nSlots][64]
 	uint32_t nTiles;
-	uint32_t rowLen;              // 2 Wp, Wp = waves padded to a multiple of 4
+	uint32_t rowLen;              // 2 Wp, Wp = waves padded to a multiple of 8
 };
 
 __device__ __forceinline__ float swap_pair(float v)
@@ -185,25 +185,30 @@ __global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventA
 				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, treg[4 * q + 3], S, 0, 0, 0);
 			}
 
+			// per register p (event 8 (p / 4) + 4 h + p % 4 of the tile, column j):
 			// sigma = |S|^2 + flat^2 (FP64), ln sigma via the running product,
-			// S <- S / sigma for the gradient; register v holds event
-			// 8 (v / 4) + 4 h + v % 4 of the tile, column j
+			// V = S / sigma; then the K step p of X[wave components][columns] +=
+			// A_tile^T V, which pairs register p of both lane halves.  Interleaving
+			// lets the VALU work of step p + 1 run under the MFMAs of step p.
 			const uint32_t evBase = (t - tileBase) * kEv;
 #pragma unroll
-			for (int v = 0; v < 16; ++v) {
-				const int row = 8 * (v >> 2) + 4 * h + (v & 3);
+			for (int p = 0; p < 16; ++p) {
+				const int row = 8 * (p >> 2) + 4 * h + (p & 3);
 				const bool valid = evBase + (uint32_t)row < nEv;
-				const float me = S[v];
+				const float me = S[p];
 				const float other = swap_pair(me);
 				const float re = cout ? other : me, im = cout ? me : other;
 				double sig = fma((double)re, (double)re, fma((double)im, (double)im, f2));
 				sig = valid ? sig : 1.;
 				const float inv = __builtin_amdgcn_rcpf((float)sig);
-				S[v] = valid ? me * inv : 0.f;
-				if ((v & 1) == cout) {  // the pair splits the event terms
+				const float vp = valid ? me * inv : 0.f;
+				if ((p & 1) == cout) {  // the pair splits the event terms
 					mant *= sig;
 					invAcc += valid ? (double)inv : 0.;
 				}
+#pragma unroll
+				for (int x = 0; x < NT; ++x)
+					X[x] = __builtin_amdgcn_mfma_f32_32x32x2f32(L[row * STRIDE + 32 * x + j], vp, X[x], 0, 0, 0);
 			}
 			{
 				// mant = m 2^e with m in [1, 2): keep m, carry e (sigma > 0 is finite or inf/nan)
@@ -213,16 +218,6 @@ __global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventA
 					expo += e - 1023;
 					mant = __longlong_as_double((long long)((u & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
 				}
-			}
-
-			// X[wave components][columns] += A_tile^T (S / sigma): K step p pairs
-			// register p of both lane halves (events 8 (p / 4) + 4 h + p % 4)
-#pragma unroll
-			for (int p = 0; p < 16; ++p) {
-				const int row = 8 * (p >> 2) + 4 * h + (p & 3);
-#pragma unroll
-				for (int x = 0; x < NT; ++x)
-					X[x] = __builtin_amdgcn_mfma_f32_32x32x2f32(L[row * STRIDE + 32 * x + j], S[p], X[x], 0, 0, 0);
 			}
 			if (more)
 				RPWA_BATCH_LSTORE(buf ^ 1);
@@ -1412,6 +1407,8 @@ extern "C" int rpwa_batch_fit(rpwa_batch* b, const double* startPar, uint32_t se
 		iterations += chunk;
 		RPWA_HIP_CHECK(hipMemcpyAsync(&nRunning, b->d_nRunning, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
 		RPWA_HIP_CHECK(hipStreamSynchronize(b->stream));
+		if (std::getenv("RPWA_VERBOSE"))
+			fprintf(stderr, "[rpwa batch] iteration %llu: %d fits running\n", (unsigned long long)iterations, nRunning);
 	}
 	b->lastEvaluations = iterations;
 
@@ -1493,5 +1490,7 @@ extern "C" int rpwa_batch_time_event_kernel(rpwa_batch* b, uint32_t reps, double
 	RPWA_HIP_CHECK(hipSetDevice(b->device));
 	std::vector<int32_t> active(b->bins.size(), 1);
 	RPWA_HIP_CHECK(hipMemcpy(b->d_binActive, active.data(), active.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+	for (int w = 0; w < 5; ++w)  // untimed: clocks and caches reach their steady state
+		RPWA_TRY(launch_event(b));
 	return time_event(b, reps, ms);
 }
