@@ -2,18 +2,23 @@
 //
 // One evaluation of all fits is four kernels, captured once in a hipGraph and
 // replayed for every L-BFGS iteration:
-//   batch_event_kernel   persistent; each workgroup streams a contiguous range
-//                        of 32-event tiles (ranges may span bins) through LDS
-//                        and, for the 64 starts of the tile's bin, computes
-//                        S = A T (FP32 MFMA), sigma, ln sigma (FP64) and
-//                        G += A^H (S / sigma) (FP32 MFMA), writing one partial
-//                        per (workgroup, bin segment);
-//   batch_reduce_kernel  per fit: partials in fixed order (FP64), norm term
-//                        T^H I T and its derivative (FP64), -lnL and gradient;
+//   batch_event_kernel   persistent waves; each wave takes work units (bin,
+//                        chunk of 64 tiles of 32 events, group of 16 starts)
+//                        and computes, tile by tile, S = A T (FP32 MFMA),
+//                        sigma, ln sigma (FP64) and G += A^H (S / sigma) (FP32
+//                        MFMA), writing one partial per unit;
+//   batch_reduce_kernel  per fit: the chunk partials in fixed order (FP64),
+//                        the norm term T^H I T and its derivative (FP64),
+//                        -lnL and gradient;
 //   batch_lbfgs_kernel   per fit: Armijo line search, L-BFGS two-loop
 //                        direction, stopping tests, next trial point;
-//   batch_status_kernel  per bin: does any start still run.
-// Reductions run in a fixed order, so results do not depend on timing.
+//   batch_plan_kernel    per bin: the running starts packed into the leading
+//                        columns, and the list of work units of the next pass
+//                        (only column groups holding a running start).
+// A partial depends only on its chunk and on the start's data, never on the
+// wave that computed it or on which column the start occupies, and the chunk
+// partials are summed in chunk order: results do not depend on timing, on the
+// grid, or on how many other fits are still running.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,31 +39,33 @@
 namespace rpwa_amd {
 namespace {
 
-constexpr int kCols = 64;      // starts per bin in one workgroup: 4 waves x 16 starts
-constexpr int kEv = 32;        // events per tile (MFMA 32x32 rows)
-constexpr int kThreads = 256;  // 4 waves
-constexpr int kMaxMem = 16;    // L-BFGS correction pairs
-constexpr int kPrefetch = 6;   // float4 per thread per tile: 32 * 192 / 4 / 256
+constexpr int kCols = 64;        // starts per bin: 4 column groups
+constexpr int kGroup = 16;       // starts per column group (one wave: 32 MFMA columns)
+constexpr int kEv = 32;          // events per tile (MFMA 32x32 rows)
+constexpr int kChunkTiles = 64;  // tiles per work unit (2048 events)
+constexpr int kThreads = 256;    // 4 independent waves per block
+constexpr int kMaxMem = 16;      // L-BFGS correction pairs
+constexpr int kTStride = kGroup + 1;  // LDS row of one wave's production amplitudes (float2)
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-// global-address-space view, so tile loads are global (not flat) loads and do
-// not share the LDS wait counter
+// global-address-space views, so amplitude loads are global (not flat) loads
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f32x4 gfloat4;
+typedef __attribute__((address_space(1))) const float gfloat;
 
 struct BatchEventArgs {
 	const float* const* binAmps;  // per bin: [tiles * 32][rowLen] float (complex64 pairs)
-	const uint32_t* tileStart;    // [nBins + 1]
+	const uint32_t* chunkBin;     // [nChunks]
+	const uint32_t* chunkTile0;   // [nChunks] first tile of the chunk within its bin
 	const uint32_t* nEvents;      // [nBins]
-	const int32_t* binActive;     // [nBins]
-	const float* T32;             // [nBins][64][rowLen]
-	const double* flat2;          // [nBins][64]
-	const uint32_t* wgSlotStart;  // [nWg]
-	float* partX;                 // [nSlots][32 NT][128]
-	double* partLn;               // [nSlots][64]
-	double* partInv;              // [nSlots][64]
-	uint32_t nTiles;
-	uint32_t rowLen;              // 2 Wp, Wp = waves padded to a multiple of 8
+	const uint32_t* units;        // [nChunks * 4] chunk * 4 + column group, this pass
+	const uint32_t* nUnits;       // [1]
+	const int32_t* colMap;        // [nBins][64] start held by each column (-1: none)
+	const float* T32;             // [nBins][64][rowLen] by start
+	const double* flat2;          // [nBins][64] by start
+	float* partG;                 // [nChunks * 4][Wp][16][2]
+	double* partLn;               // [nChunks * 4][16]
+	double* partInv;              // [nChunks * 4][16]
 };
 
 __device__ __forceinline__ float swap_pair(float v)
@@ -75,64 +82,54 @@ __device__ __forceinline__ double swap_pair_d(double v)
 	return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+// One wave per work unit.  Lane (h, j): MFMA column j = (start slot j / 2 of
+// the group, Re / Im of S by j % 2); MFMA rows are events (S = A T) or wave
+// components (G = A^T V).  KQ = Wp / 4.
 template <int KQ>
-__global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventArgs a)
+__global__ __launch_bounds__(kThreads, 2) void batch_event_kernel(const BatchEventArgs a)
 {
-	constexpr int NT = (KQ + 3) / 4;   // tiles of 32 wave components
-	constexpr int XROWS = 32 * NT;     // padded wave-component rows of the gradient partial
-	constexpr int STRIDE = XROWS + 4;  // LDS row stride [floats]: rows 16 bytes apart in the banks
-	__shared__ __attribute__((aligned(16))) float lds[2 * kEv * STRIDE];
+	constexpr int NT = (KQ + 3) / 4;            // tiles of 32 wave components
+	constexpr uint32_t rowLen = 8 * KQ;         // 2 Wp floats per event
+	constexpr uint32_t Wp = 4 * KQ;
+	constexpr int lastCols = (int)rowLen - 32 * (NT - 1);  // valid columns of the last component tile
 
-	const int tid = threadIdx.x;
-	const int wave = tid >> 6, lane = tid & 63;
+	__shared__ float2 Tlds[kThreads / 64][Wp * kTStride];
+	const int lane = threadIdx.x & 63;
 	const int h = lane >> 5, j = lane & 31;
-	const int cout = j & 1;                    // this lane's column: Re (0) or Im (1) of S
-	const int start = 16 * wave + (j >> 1);    // this lane's fit start
-	constexpr uint32_t rowLen = 8 * KQ;        // 2 Wp floats per event
-	constexpr uint32_t kSteps = 4 * KQ;        // MFMA K steps of S = A T
-	constexpr int nF4 = kEv * (int)rowLen / 4; // float4 per tile
+	const int cout = j & 1;                     // this lane's column: Re (0) or Im (1) of S
+	float2* Tw = Tlds[threadIdx.x >> 6];
+	const uint32_t nWaves = gridDim.x * (kThreads / 64);
+	const uint32_t nUnits = *a.nUnits;
 
-	// padding columns [rowLen, STRIDE) are never written by tile loads
-	for (int idx = tid; idx < 2 * kEv * STRIDE; idx += kThreads)
-		if ((uint32_t)(idx % STRIDE) >= rowLen)
-			lds[idx] = 0.f;
-	__syncthreads();
-
-	const uint32_t G = gridDim.x, g = blockIdx.x;
-	const uint32_t t0 = (uint32_t)(((uint64_t)g * a.nTiles) / G);
-	const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * a.nTiles) / G);
-	if (t0 >= t1)
-		return;
-	uint32_t b = 0;
-	while (a.tileStart[b + 1] <= t0)
-		++b;
-	uint32_t slot = a.wgSlotStart[g];
-
-	for (uint32_t t = t0; t < t1; ++slot, ++b) {
-		const uint32_t segEnd = min(t1, a.tileStart[b + 1]);
-		if (!a.binActive[b]) {
-			t = segEnd;
-			continue;
-		}
-		const float* amps = a.binAmps[b];
-		const uint32_t tileBase = a.tileStart[b];
+	for (uint32_t u = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); u < nUnits; u += nWaves) {
+		const uint32_t unit = a.units[u];
+		const uint32_t cg = unit >> 2, q = unit & 3;
+		const uint32_t b = a.chunkBin[cg];
 		const uint32_t nEv = a.nEvents[b];
+		const uint32_t tile0 = a.chunkTile0[cg];
+		const uint32_t tile1 = min(tile0 + (uint32_t)kChunkTiles, (nEv + kEv - 1) / kEv);
+		const int start = a.colMap[b * kCols + kGroup * q + (j >> 1)];
 
-		// B operand of S = A T in registers: K step k, lane half h -> wave
-		// component wc = 8 (k / 4) + 4 h + k % 4 (matches the float4 LDS reads)
-		float treg[kSteps];
+		// production amplitudes of the group's 16 starts in this wave's LDS
+		// region, [wave i][start slot] (Re, Im); padding columns are zero
 		{
-			const float* Tb = a.T32 + ((size_t)b * kCols + start) * rowLen;
-#pragma unroll
-			for (int k = 0; k < (int)kSteps; ++k) {
-				const int wc = 8 * (k >> 2) + 4 * h + (k & 3);
-				const int cin = wc & 1;
-				const float tre = Tb[wc - cin], tim = Tb[wc - cin + 1];
-				// [Sre Sim] = [Are Aim] [[Tre Tim] [-Tim Tre]]
-				treg[k] = (cin == cout) ? tre : (cin == 0 ? tim : -tim);
+			const int* cm = a.colMap + b * kCols + kGroup * q;
+			for (int idx = lane; idx < (int)Wp * kGroup; idx += 64) {
+				const int i = idx / kGroup, sl = idx % kGroup;
+				const int st = cm[sl];
+				float2 v = make_float2(0.f, 0.f);
+				if (st >= 0) {
+					const float* Tb = a.T32 + ((size_t)b * kCols + st) * rowLen;
+					v = make_float2(Tb[2 * i], Tb[2 * i + 1]);
+				}
+				Tw[i * kTStride + sl] = v;
 			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		}
-		const double f2 = a.flat2[(size_t)b * kCols + start];
+		const double f2 = start >= 0 ? a.flat2[(size_t)b * kCols + start] : 1.;
+		const float2* Tj = Tw + (j >> 1);
 
 		f32x16 X[NT];
 #pragma unroll
@@ -141,58 +138,68 @@ __global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventA
 		double mant = 1.;  // running product of sigma, renormalised per tile
 		int expo = 0;
 		double invAcc = 0.;
+		const float* amps = a.binAmps[b];
 
-		f32x4 pf[kPrefetch];
-#define RPWA_BATCH_GLOAD(tile)                                                                              \
-	do {                                                                                                    \
-		const gfloat4* src_ = (const gfloat4*)(amps + (size_t)((tile) - tileBase) * kEv * rowLen);     \
-		_Pragma("unroll") for (int r_ = 0; r_ < kPrefetch; ++r_)                                            \
-		{                                                                                                   \
-			const int idx_ = min(tid + kThreads * r_, nF4 - 1);                                             \
-			pf[r_] = src_[idx_];                                                                            \
-		}                                                                                                   \
-	} while (0)
-#define RPWA_BATCH_LSTORE(buf)                                                                              \
-	do {                                                                                                    \
-		_Pragma("unroll") for (int r_ = 0; r_ < kPrefetch; ++r_)                                            \
-		{                                                                                                   \
-			const int idx_ = tid + kThreads * r_;                                                           \
-			if (idx_ < nF4) {                                                                               \
-				const int e_ = (4 * idx_) / (int)rowLen, c_ = (4 * idx_) % (int)rowLen;                     \
-				*reinterpret_cast<f32x4*>(&lds[(buf) * kEv * STRIDE + e_ * STRIDE + c_]) = pf[r_];         \
-			}                                                                                               \
-		}                                                                                                   \
-	} while (0)
-
-		RPWA_BATCH_GLOAD(t);
-		RPWA_BATCH_LSTORE(0);
-		__syncthreads();
-		int buf = 0;
-		for (; t < segEnd; ++t) {
-			const bool more = t + 1 < segEnd;
-			if (more)
-				RPWA_BATCH_GLOAD(t + 1);
-			const float* L = lds + buf * kEv * STRIDE;
-
-			// S[32 events][32 columns of this wave] = A_tile T
+		// Operands are loaded kPd1 steps (GEMM1) and kPd2 steps (GEMM2) ahead; the
+		// scheduling barriers keep the compiler from hoisting every load of a
+		// tile (register pressure).
+		constexpr int kPd2 = 2;
+		constexpr int kPd1 = KQ < 4 ? KQ : 4;
+		for (uint32_t t = tile0; t < tile1; ++t) {
+			// S[32 events][32 columns] = A_tile T
+			const gfloat4* rowA = (const gfloat4*)(amps + ((size_t)t * kEv + j) * rowLen);
 			f32x16 S = f32x16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+			f32x4 avBuf[kPd1];
 #pragma unroll
-			for (int q = 0; q < KQ; ++q) {
-				const float4 av = *reinterpret_cast<const float4*>(&L[j * STRIDE + 8 * q + 4 * h]);
-				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, treg[4 * q + 0], S, 0, 0, 0);
-				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, treg[4 * q + 1], S, 0, 0, 0);
-				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, treg[4 * q + 2], S, 0, 0, 0);
-				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, treg[4 * q + 3], S, 0, 0, 0);
+			for (int qq = 0; qq < kPd1; ++qq)
+				avBuf[qq] = rowA[2 * qq + h];
+			float2 t0n = Tj[(2 * h) * kTStride], t1n = Tj[(2 * h + 1) * kTStride];
+#pragma unroll
+			for (int q4 = 0; q4 < KQ; ++q4) {
+				// K steps 4 q4 .. 4 q4 + 3 of lane half h are the wave components
+				// 8 q4 + 4 h + 0..3, i.e. (Re, Im) of waves i0 = 4 q4 + 2 h and i0 + 1;
+				// [Sre Sim] = [Are Aim] [[Tre Tim] [-Tim Tre]]
+				const f32x4 av = avBuf[q4 % kPd1];
+				if (q4 + kPd1 < KQ)
+					avBuf[q4 % kPd1] = rowA[2 * (q4 + kPd1) + h];
+				const float2 t0 = t0n, t1 = t1n;
+				if (q4 + 1 < KQ) {
+					t0n = Tj[(4 * (q4 + 1) + 2 * h) * kTStride];
+					t1n = Tj[(4 * (q4 + 1) + 2 * h + 1) * kTStride];
+				}
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, cout ? t0.y : t0.x, S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, cout ? t0.x : -t0.y, S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, cout ? t1.y : t1.x, S, 0, 0, 0);
+				S = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, cout ? t1.x : -t1.y, S, 0, 0, 0);
+				__builtin_amdgcn_sched_barrier(0);
 			}
-
 			// per register p (event 8 (p / 4) + 4 h + p % 4 of the tile, column j):
 			// sigma = |S|^2 + flat^2 (FP64), ln sigma via the running product,
-			// V = S / sigma; then the K step p of X[wave components][columns] +=
-			// A_tile^T V, which pairs register p of both lane halves.  Interleaving
-			// lets the VALU work of step p + 1 run under the MFMAs of step p.
-			const uint32_t evBase = (t - tileBase) * kEv;
+			// V = S / sigma; then K step p of X[wave components][columns] +=
+			// A_tile^T V, which pairs register p of both lane halves.  Columns of
+			// the last component tile past 2 Wp read within the row's tile and are
+			// zeroed.
+			const uint32_t evBase = t * kEv;
+			const gfloat* tileB = (const gfloat*)(amps + ((size_t)t * kEv + 4 * h) * rowLen);
+			float a2Buf[kPd2][NT];
+#pragma unroll
+			for (int pp = 0; pp < kPd2; ++pp)
+#pragma unroll
+				for (int x = 0; x < NT; ++x)
+					a2Buf[pp][x] = tileB[(8 * (pp >> 2) + (pp & 3)) * rowLen + 32 * x + (x < NT - 1 ? j : min(j, lastCols - 1))];
 #pragma unroll
 			for (int p = 0; p < 16; ++p) {
+				float a2[NT];
+#pragma unroll
+				for (int x = 0; x < NT; ++x)
+					a2[x] = (x < NT - 1 || lastCols >= 32 || j < lastCols) ? a2Buf[p % kPd2][x] : 0.f;
+				if (p + kPd2 < 16) {
+					const int pn = p + kPd2;
+					const gfloat* rowB = tileB + (8 * (pn >> 2) + (pn & 3)) * rowLen;
+#pragma unroll
+					for (int x = 0; x < NT; ++x)
+						a2Buf[p % kPd2][x] = rowB[32 * x + (x < NT - 1 ? j : min(j, lastCols - 1))];
+				}
 				const int row = 8 * (p >> 2) + 4 * h + (p & 3);
 				const bool valid = evBase + (uint32_t)row < nEv;
 				const float me = S[p];
@@ -208,31 +215,37 @@ __global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventA
 				}
 #pragma unroll
 				for (int x = 0; x < NT; ++x)
-					X[x] = __builtin_amdgcn_mfma_f32_32x32x2f32(L[row * STRIDE + 32 * x + j], vp, X[x], 0, 0, 0);
+					X[x] = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[x], vp, X[x], 0, 0, 0);
+				__builtin_amdgcn_sched_barrier(0);
 			}
 			{
 				// mant = m 2^e with m in [1, 2): keep m, carry e (sigma > 0 is finite or inf/nan)
-				const uint64_t u = (uint64_t)__double_as_longlong(mant);
-				const int e = (int)((u >> 52) & 0x7FF);
+				const uint64_t ub = (uint64_t)__double_as_longlong(mant);
+				const int e = (int)((ub >> 52) & 0x7FF);
 				if (e != 0 && e != 0x7FF) {
 					expo += e - 1023;
-					mant = __longlong_as_double((long long)((u & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+					mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
 				}
 			}
-			if (more)
-				RPWA_BATCH_LSTORE(buf ^ 1);
-			__syncthreads();
-			buf ^= 1;
 		}
 
-		// partial of this (workgroup, bin segment)
-		float* px = a.partX + (size_t)slot * XROWS * 128;
+		// partial of this unit: G_i = sum conj(A_i) V per start slot.  Rows 2i,
+		// 2i+1 of X are registers v, v+1; columns 2s, 2s+1 are lanes j, j^1:
+		// Re G = X[2i][2s] + X[2i+1][2s+1], Im G = X[2i][2s+1] - X[2i+1][2s]
+		const uint32_t slot = cg * 4 + q;
+		float* pg = a.partG + (size_t)slot * Wp * (2 * kGroup);
 #pragma unroll
 		for (int x = 0; x < NT; ++x)
 #pragma unroll
-			for (int v = 0; v < 16; ++v) {
-				const int wc = 32 * x + 8 * (v >> 2) + 4 * h + (v & 3);
-				px[(size_t)wc * 128 + 32 * wave + j] = X[x][v];
+			for (int v = 0; v < 16; v += 2) {
+				const int row0 = 32 * x + 8 * (v >> 2) + 4 * h + (v & 3);
+				const float own0 = X[x][v], own1 = X[x][v + 1];
+				const float p0 = swap_pair(own0), p1 = swap_pair(own1);
+				if (cout == 0 && row0 < (int)rowLen) {
+					const int i = row0 >> 1;
+					pg[(i * kGroup + (j >> 1)) * 2 + 0] = own0 + p1;
+					pg[(i * kGroup + (j >> 1)) * 2 + 1] = p0 - own1;
+				}
 			}
 		double lnv = log(mant) + (double)expo * 0.69314718055994530942;
 		lnv += swap_pair_d(lnv);
@@ -240,14 +253,11 @@ __global__ __launch_bounds__(kThreads) void batch_event_kernel(const BatchEventA
 		double inv = invAcc + swap_pair_d(invAcc);
 		inv += __shfl_xor(inv, 32);
 		if (h == 0 && cout == 0) {
-			a.partLn[(size_t)slot * kCols + start] = lnv;
-			a.partInv[(size_t)slot * kCols + start] = inv;
+			a.partLn[(size_t)slot * kGroup + (j >> 1)] = lnv;
+			a.partInv[(size_t)slot * kGroup + (j >> 1)] = inv;
 		}
 	}
 }
-
-#undef RPWA_BATCH_GLOAD
-#undef RPWA_BATCH_LSTORE
 
 typedef void (*BatchEventKernel)(const BatchEventArgs);
 
@@ -305,26 +315,29 @@ constexpr int ST_PAD = -100;  // padding fit (start >= nStarts): never runs
 __device__ __forceinline__ bool running(int st) { return st == ST_INIT || st == ST_SEARCH; }
 
 struct BatchReduceArgs {
-	const float* partX;
+	const float* partG;
 	const double* partLn;
 	const double* partInv;
-	const uint32_t* binSlot;  // [nBins + 1]
-	const int32_t* binActive;
+	const uint32_t* binChunk0;  // [nBins + 1]
+	const int32_t* startCol;    // [nBins][64] column of each start this pass (-1: not evaluated)
 	const float* T32;
-	const double* acc;        // [nBins][W][W] complex, normalised
-	const double* totAcc;     // [nBins]
+	const double* acc;          // [nBins][W][W] complex, normalised
+	const double* totAcc;       // [nBins]
 	const double* xt;
 	double* ft;
 	double* gt;
-	uint32_t W, P, rowLen, nStarts, XROWS;
+	uint32_t W, Wp, P, rowLen;
 };
 
-// per fit: gradient partials in slot order, norm term, value and gradient
+// per fit: gradient partials in chunk order, norm term, value and gradient
 __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs r)
 {
-	const uint32_t b = blockIdx.x / kCols, s = blockIdx.x % kCols;
-	if (!r.binActive[b] || s >= r.nStarts)
+	const uint32_t b = blockIdx.x / kCols;
+	const int col = r.startCol[blockIdx.x];
+	if (col < 0)
 		return;
+	const uint32_t q = (uint32_t)col / kGroup, jc = (uint32_t)col % kGroup;
+	const uint32_t c0 = r.binChunk0[b], c1 = r.binChunk0[b + 1];
 	const size_t fit = blockIdx.x;
 	const uint32_t i = threadIdx.x;
 	__shared__ double red[128];
@@ -332,12 +345,10 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 	double normPart = 0.;
 	if (i < r.W) {
 		double gre = 0., gim = 0.;
-		for (uint32_t sl = r.binSlot[b]; sl < r.binSlot[b + 1]; ++sl) {
-			const float* X = r.partX + (size_t)sl * r.XROWS * 128;
-			const double x00 = X[(size_t)(2 * i) * 128 + 2 * s], x11 = X[(size_t)(2 * i + 1) * 128 + 2 * s + 1];
-			const double x01 = X[(size_t)(2 * i) * 128 + 2 * s + 1], x10 = X[(size_t)(2 * i + 1) * 128 + 2 * s];
-			gre += x00 + x11;
-			gim += x01 - x10;
+		for (uint32_t c = c0; c < c1; ++c) {
+			const float* pg = r.partG + ((size_t)(c * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
+			gre += (double)pg[0];
+			gim += (double)pg[1];
 		}
 		// u_i = sum_j T_j conj(I_ij); norm = sum_i Re(T_i conj(u_i)); d norm / d T_i = 2 u_i
 		const double* I = r.acc + ((size_t)b * r.W + i) * r.W * 2;
@@ -368,9 +379,9 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 	}
 	if (i == 0) {
 		double lnSum = 0., invSum = 0.;
-		for (uint32_t sl = r.binSlot[b]; sl < r.binSlot[b + 1]; ++sl) {
-			lnSum += r.partLn[(size_t)sl * kCols + s];
-			invSum += r.partInv[(size_t)sl * kCols + s];
+		for (uint32_t c = c0; c < c1; ++c) {
+			lnSum += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
+			invSum += r.partInv[(size_t)(c * 4 + q) * kGroup + jc];
 		}
 		const double flat = r.xt[fit * r.P + r.P - 1];
 		const double eta = r.totAcc[b];
@@ -638,24 +649,39 @@ __global__ __launch_bounds__(64) void batch_lbfgs_kernel(const BatchLbfgsArgs a)
 	}
 }
 
-struct BatchStatusArgs {
-	const int32_t* status;
-	int32_t* binActive;
+struct BatchPlanArgs {
+	const int32_t* status;      // [nBins][64]
+	const uint32_t* binChunk0;  // [nBins + 1]
+	int32_t* colMap;            // [nBins][64]
+	int32_t* startCol;          // [nBins][64]
+	uint32_t* unitCount;        // [nBins] scratch
+	uint32_t* units;            // [nChunks * 4]
+	uint32_t* nUnits;
 	int32_t* nRunning;
-	uint32_t nBins;
+	uint32_t nBins, nStarts;
+	int evalAll;                // 1: every start < nStarts is evaluated (no minimiser state)
 };
 
-// one block: bins with a running start, and the number of running fits
-__global__ __launch_bounds__(256) void batch_status_kernel(const BatchStatusArgs a)
+// one block: per bin the evaluated starts packed into the leading columns (in
+// start order), the column groups that hold them, and the work units of the
+// next pass in (bin, chunk, group) order; the number of running fits
+__global__ __launch_bounds__(256) void batch_plan_kernel(const BatchPlanArgs a)
 {
 	__shared__ int32_t red[256];
 	int32_t count = 0;
 	for (uint32_t b = threadIdx.x; b < a.nBins; b += 256) {
-		int32_t n = 0;
-		for (int s = 0; s < kCols; ++s)
-			n += running(a.status[(size_t)b * kCols + s]) ? 1 : 0;
-		a.binActive[b] = n > 0 ? 1 : 0;
-		count += n;
+		int k = 0;
+		for (int s = 0; s < kCols; ++s) {
+			const bool run = a.evalAll ? (uint32_t)s < a.nStarts : running(a.status[(size_t)b * kCols + s]);
+			a.startCol[(size_t)b * kCols + s] = run ? k : -1;
+			if (run)
+				a.colMap[(size_t)b * kCols + k++] = s;
+		}
+		for (int c = k; c < kCols; ++c)
+			a.colMap[(size_t)b * kCols + c] = -1;
+		const uint32_t groups = (uint32_t)(k + kGroup - 1) / kGroup;
+		a.unitCount[b] = groups * (a.binChunk0[b + 1] - a.binChunk0[b]);
+		count += a.evalAll ? 0 : k;
 	}
 	red[threadIdx.x] = count;
 	__syncthreads();
@@ -666,6 +692,29 @@ __global__ __launch_bounds__(256) void batch_status_kernel(const BatchStatusArgs
 	}
 	if (threadIdx.x == 0)
 		*a.nRunning = red[0];
+	// unit offsets per bin (sequential: bins are few), then the units
+	__shared__ uint32_t total;
+	if (threadIdx.x == 0) {
+		uint32_t off = 0;
+		for (uint32_t b = 0; b < a.nBins; ++b) {
+			const uint32_t n = a.unitCount[b];
+			a.unitCount[b] = off;
+			off += n;
+		}
+		total = off;
+		*a.nUnits = off;
+	}
+	__syncthreads();
+	for (uint32_t b = 0; b < a.nBins; ++b) {
+		const uint32_t off = a.unitCount[b];
+		const uint32_t n = (b + 1 < a.nBins ? a.unitCount[b + 1] : total) - off;
+		const uint32_t chunks = a.binChunk0[b + 1] - a.binChunk0[b];
+		const uint32_t groups = chunks ? n / chunks : 0;
+		for (uint32_t k = threadIdx.x; k < n; k += 256) {
+			const uint32_t c = k / groups, g = k % groups;
+			a.units[off + k] = (a.binChunk0[b] + c) * 4 + g;
+		}
+	}
 }
 
 struct BatchInitArgs {
@@ -741,24 +790,29 @@ struct rpwa_batch {
 	};
 	std::vector<Bin> bins;
 	bool finished = false;
-	uint32_t nTiles = 0, nSlots = 0, nWg = 0, wgPerCu = 0, wgPerCuFixed = 0, maxWgPerCu = 1;
+	uint32_t nTiles = 0, nChunks = 0, nWg = 0, wgPerCu = 0, wgPerCuFixed = 0, maxWgPerCu = 1;
 	uint64_t totalEvents = 0;
 	uint64_t lastEvaluations = 0;
 	// device
 	float** d_binAmps = nullptr;
-	uint32_t* d_tileStart = nullptr;
+	uint32_t* d_chunkBin = nullptr;
+	uint32_t* d_chunkTile0 = nullptr;
+	uint32_t* d_binChunk0 = nullptr;
 	uint32_t* d_nEvents = nullptr;
-	int32_t* d_binActive = nullptr;
-	uint32_t* d_wgSlotStart = nullptr;
-	uint32_t* d_binSlot = nullptr;
+	uint32_t* d_units = nullptr;
+	uint32_t* d_nUnits = nullptr;
+	uint32_t* d_unitCount = nullptr;
+	int32_t* d_colMap = nullptr;
+	int32_t* d_startCol = nullptr;
 	float* d_T32 = nullptr;
 	double* d_flat2 = nullptr;
-	float* d_partX = nullptr;
+	float* d_partG = nullptr;
 	double* d_partLn = nullptr;
 	double* d_partInv = nullptr;
 	double* d_acc = nullptr;
 	double* d_totAcc = nullptr;
 	int32_t* d_nRunning = nullptr;
+	int32_t* d_noStatus = nullptr;  // all zero: status array of the evaluation path
 	FitState st{};
 	uint32_t memory = 0;  // history depth the state was allocated for
 	std::vector<void*> allocations;
@@ -801,82 +855,32 @@ int dalloc(rpwa_batch* b, T** p, size_t n)
 
 size_t nFits(const rpwa_batch* b) { return b->bins.size() * kCols; }
 
-// static plan: workgroup g streams tiles [g T / G, (g + 1) T / G); a slot per
-// (workgroup, bin segment); each bin's slots are contiguous and in workgroup order
-void make_plan(const rpwa_batch* b, uint32_t nWg, std::vector<uint32_t>& wgSlotStart, std::vector<uint32_t>& binSlot,
-               uint32_t& nSlots)
-{
-	const uint32_t nBins = (uint32_t)b->bins.size();
-	std::vector<uint32_t> tileStart(nBins + 1, 0);
-	for (uint32_t i = 0; i < nBins; ++i)
-		tileStart[i + 1] = tileStart[i] + b->bins[i].nTiles;
-	const uint32_t T = tileStart[nBins];
-	wgSlotStart.assign(nWg, 0);
-	std::vector<std::vector<uint32_t> > slotsOfBin(nBins);
-	uint32_t slot = 0;
-	for (uint32_t g = 0; g < nWg; ++g) {
-		const uint32_t t0 = (uint32_t)(((uint64_t)g * T) / nWg), t1 = (uint32_t)(((uint64_t)(g + 1) * T) / nWg);
-		wgSlotStart[g] = slot;
-		if (t0 >= t1)
-			continue;
-		uint32_t bi = 0;
-		while (tileStart[bi + 1] <= t0)
-			++bi;
-		for (uint32_t t = t0; t < t1; ++bi) {
-			slotsOfBin[bi].push_back(slot++);
-			t = std::min(t1, tileStart[bi + 1]);
-		}
-	}
-	nSlots = slot;
-	binSlot.assign(nBins + 1, 0);
-	for (uint32_t i = 0; i < nBins; ++i) {
-		// contiguous by construction: workgroups are visited in order
-		binSlot[i] = slotsOfBin[i].front();
-		binSlot[i + 1] = slotsOfBin[i].back() + 1;
-	}
-}
-
-int upload_plan(rpwa_batch* b, uint32_t nWg)
-{
-	std::vector<uint32_t> wgSlotStart, binSlot;
-	uint32_t nSlots;
-	make_plan(b, nWg, wgSlotStart, binSlot, nSlots);
-	if (nSlots > b->nSlots) {
-		set_error("rpwa_batch: internal error, partial slots exceed the allocation");
-		return RPWA_ERR_STATE;
-	}
-	RPWA_HIP_CHECK(hipMemcpy(b->d_wgSlotStart, wgSlotStart.data(), nWg * sizeof(uint32_t), hipMemcpyHostToDevice));
-	RPWA_HIP_CHECK(hipMemcpy(b->d_binSlot, binSlot.data(), binSlot.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-	b->nWg = nWg;
-	return RPWA_OK;
-}
-
 BatchEventArgs event_args(const rpwa_batch* b)
 {
 	BatchEventArgs a;
 	a.binAmps = b->d_binAmps;
-	a.tileStart = b->d_tileStart;
+	a.chunkBin = b->d_chunkBin;
+	a.chunkTile0 = b->d_chunkTile0;
 	a.nEvents = b->d_nEvents;
-	a.binActive = b->d_binActive;
+	a.units = b->d_units;
+	a.nUnits = b->d_nUnits;
+	a.colMap = b->d_colMap;
 	a.T32 = b->d_T32;
 	a.flat2 = b->d_flat2;
-	a.wgSlotStart = b->d_wgSlotStart;
-	a.partX = b->d_partX;
+	a.partG = b->d_partG;
 	a.partLn = b->d_partLn;
 	a.partInv = b->d_partInv;
-	a.nTiles = b->nTiles;
-	a.rowLen = b->rowLen;
 	return a;
 }
 
 BatchReduceArgs reduce_args(const rpwa_batch* b)
 {
 	BatchReduceArgs r;
-	r.partX = b->d_partX;
+	r.partG = b->d_partG;
 	r.partLn = b->d_partLn;
 	r.partInv = b->d_partInv;
-	r.binSlot = b->d_binSlot;
-	r.binActive = b->d_binActive;
+	r.binChunk0 = b->d_binChunk0;
+	r.startCol = b->d_startCol;
 	r.T32 = b->d_T32;
 	r.acc = b->d_acc;
 	r.totAcc = b->d_totAcc;
@@ -884,16 +888,16 @@ BatchReduceArgs reduce_args(const rpwa_batch* b)
 	r.ft = b->st.ft;
 	r.gt = b->st.gt;
 	r.W = b->W;
+	r.Wp = b->Wp;
 	r.P = b->P;
 	r.rowLen = b->rowLen;
-	r.nStarts = b->nStarts;
-	r.XROWS = 32 * b->NT;
 	return r;
 }
 
 int launch_event(rpwa_batch* b)
 {
-	hipLaunchKernelGGL(pick_event_kernel((int)(b->Wp / 4)), dim3(b->nWg), dim3(kThreads), 0, b->stream, event_args(b));
+	hipLaunchKernelGGL(pick_event_kernel((int)(b->Wp / 4)), dim3(b->nWg), dim3(kThreads), 0, b->stream,
+	                   event_args(b));
 	RPWA_HIP_CHECK(hipGetLastError());
 	return RPWA_OK;
 }
@@ -905,14 +909,22 @@ int launch_reduce(rpwa_batch* b)
 	return RPWA_OK;
 }
 
-int launch_status(rpwa_batch* b)
+// evalAll: plan for an evaluation of every start (no minimiser state)
+int launch_plan(rpwa_batch* b, bool evalAll)
 {
-	BatchStatusArgs s;
-	s.status = b->st.status;
-	s.binActive = b->d_binActive;
-	s.nRunning = b->d_nRunning;
-	s.nBins = (uint32_t)b->bins.size();
-	hipLaunchKernelGGL(batch_status_kernel, dim3(1), dim3(256), 0, b->stream, s);
+	BatchPlanArgs a;
+	a.status = evalAll ? b->d_noStatus : b->st.status;
+	a.binChunk0 = b->d_binChunk0;
+	a.colMap = b->d_colMap;
+	a.startCol = b->d_startCol;
+	a.unitCount = b->d_unitCount;
+	a.units = b->d_units;
+	a.nUnits = b->d_nUnits;
+	a.nRunning = b->d_nRunning;
+	a.nBins = (uint32_t)b->bins.size();
+	a.nStarts = b->nStarts;
+	a.evalAll = evalAll ? 1 : 0;
+	hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(256), 0, b->stream, a);
 	RPWA_HIP_CHECK(hipGetLastError());
 	return RPWA_OK;
 }
@@ -944,11 +956,6 @@ int alloc_state(rpwa_batch* b, uint32_t memory)
 	RPWA_TRY(dalloc(b, &s.x, F * P));
 	RPWA_TRY(dalloc(b, &s.g, F * P));
 	RPWA_TRY(dalloc(b, &s.d, F * P));
-	if (!s.xt) {
-		RPWA_TRY(dalloc(b, &s.xt, F * P));
-		RPWA_TRY(dalloc(b, &s.gt, F * P));
-		RPWA_TRY(dalloc(b, &s.ft, F));
-	}
 	RPWA_TRY(dalloc(b, &s.S, F * memory * P));
 	RPWA_TRY(dalloc(b, &s.Y, F * memory * P));
 	RPWA_TRY(dalloc(b, &s.rho, F * memory));
@@ -1011,6 +1018,17 @@ void host_start_values(const rpwa_batch* b, uint32_t seed, std::vector<double>& 
 			}
 		}
 	}
+}
+
+// trial points (padded [nBins * 64][P]) -> T32 / flat^2, plan of an evaluation of every start
+int prepare_eval(rpwa_batch* b, const std::vector<double>& padded)
+{
+	RPWA_HIP_CHECK(hipMemcpyAsync(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice,
+	                              b->stream));
+	BatchPrepArgs pa{b->st.xt, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
+	hipLaunchKernelGGL(batch_prepare_kernel, dim3((uint32_t)nFits(b)), dim3(64), 0, b->stream, pa);
+	RPWA_HIP_CHECK(hipGetLastError());
+	return launch_plan(b, true);
 }
 
 }  // namespace
@@ -1179,70 +1197,84 @@ extern "C" int rpwa_batch_finish(rpwa_batch* b)
 	}
 	RPWA_HIP_CHECK(hipSetDevice(b->device));
 	const uint32_t nBins = (uint32_t)b->bins.size();
-	std::vector<uint32_t> tileStart(nBins + 1, 0), nEv(nBins);
+	// work chunks: each bin's tiles in runs of kChunkTiles
+	std::vector<uint32_t> binChunk0(nBins + 1, 0), nEv(nBins), chunkBin, chunkTile0;
 	std::vector<float*> ampPtr(nBins);
 	std::vector<double> acc, totAcc(nBins);
+	b->nTiles = 0;
 	for (uint32_t i = 0; i < nBins; ++i) {
-		tileStart[i + 1] = tileStart[i] + b->bins[i].nTiles;
-		nEv[i] = (uint32_t)b->bins[i].nEvents;
-		ampPtr[i] = b->bins[i].d_amps;
-		acc.insert(acc.end(), b->bins[i].acc.begin(), b->bins[i].acc.end());
-		totAcc[i] = b->bins[i].totAcc;
+		const auto& bin = b->bins[i];
+		const uint32_t chunks = (bin.nTiles + kChunkTiles - 1) / kChunkTiles;
+		binChunk0[i + 1] = binChunk0[i] + chunks;
+		for (uint32_t c = 0; c < chunks; ++c) {
+			chunkBin.push_back(i);
+			chunkTile0.push_back(c * kChunkTiles);
+		}
+		b->nTiles += bin.nTiles;
+		nEv[i] = (uint32_t)bin.nEvents;
+		ampPtr[i] = bin.d_amps;
+		acc.insert(acc.end(), bin.acc.begin(), bin.acc.end());
+		totAcc[i] = bin.totAcc;
 	}
-	b->nTiles = tileStart[nBins];
+	b->nChunks = binChunk0[nBins];
 	int occ = 0;
-	RPWA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_event_kernel((int)(b->Wp / 4)), kThreads, 0));
+	RPWA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_event_kernel((int)(b->Wp / 4)), kThreads,
+	                                                            0));
 	b->maxWgPerCu = (uint32_t)std::max(1, std::min(occ, 4));
-	b->nSlots = b->nCu * b->maxWgPerCu + nBins;
-	const size_t F = nFits(b);
+	const size_t F = nFits(b), slots = (size_t)b->nChunks * 4;
 	RPWA_TRY(dalloc(b, &b->d_binAmps, nBins));
-	RPWA_TRY(dalloc(b, &b->d_tileStart, nBins + 1));
+	RPWA_TRY(dalloc(b, &b->d_chunkBin, b->nChunks));
+	RPWA_TRY(dalloc(b, &b->d_chunkTile0, b->nChunks));
+	RPWA_TRY(dalloc(b, &b->d_binChunk0, nBins + 1));
 	RPWA_TRY(dalloc(b, &b->d_nEvents, nBins));
-	RPWA_TRY(dalloc(b, &b->d_binActive, nBins));
-	RPWA_TRY(dalloc(b, &b->d_wgSlotStart, (size_t)b->nCu * b->maxWgPerCu));
-	RPWA_TRY(dalloc(b, &b->d_binSlot, nBins + 1));
+	RPWA_TRY(dalloc(b, &b->d_units, slots));
+	RPWA_TRY(dalloc(b, &b->d_nUnits, 1));
+	RPWA_TRY(dalloc(b, &b->d_unitCount, nBins));
+	RPWA_TRY(dalloc(b, &b->d_colMap, F));
+	RPWA_TRY(dalloc(b, &b->d_startCol, F));
 	RPWA_TRY(dalloc(b, &b->d_T32, F * b->rowLen));
 	RPWA_TRY(dalloc(b, &b->d_flat2, F));
-	RPWA_TRY(dalloc(b, &b->d_partX, (size_t)b->nSlots * 32 * b->NT * 128));
-	RPWA_TRY(dalloc(b, &b->d_partLn, (size_t)b->nSlots * kCols));
-	RPWA_TRY(dalloc(b, &b->d_partInv, (size_t)b->nSlots * kCols));
+	RPWA_TRY(dalloc(b, &b->d_partG, slots * b->Wp * 2 * kGroup));
+	RPWA_TRY(dalloc(b, &b->d_partLn, slots * kGroup));
+	RPWA_TRY(dalloc(b, &b->d_partInv, slots * kGroup));
 	RPWA_TRY(dalloc(b, &b->d_acc, acc.size()));
 	RPWA_TRY(dalloc(b, &b->d_totAcc, nBins));
 	RPWA_TRY(dalloc(b, &b->d_nRunning, 1));
+	RPWA_TRY(dalloc(b, &b->d_noStatus, F));
 	RPWA_TRY(dalloc(b, &b->st.xt, F * b->P));
 	RPWA_TRY(dalloc(b, &b->st.gt, F * b->P));
 	RPWA_TRY(dalloc(b, &b->st.ft, F));
 	RPWA_HIP_CHECK(hipMemcpy(b->d_binAmps, ampPtr.data(), nBins * sizeof(float*), hipMemcpyHostToDevice));
-	RPWA_HIP_CHECK(hipMemcpy(b->d_tileStart, tileStart.data(), (nBins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_chunkBin, chunkBin.data(), b->nChunks * sizeof(uint32_t), hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_chunkTile0, chunkTile0.data(), b->nChunks * sizeof(uint32_t),
+	                         hipMemcpyHostToDevice));
+	RPWA_HIP_CHECK(hipMemcpy(b->d_binChunk0, binChunk0.data(), (nBins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
 	RPWA_HIP_CHECK(hipMemcpy(b->d_nEvents, nEv.data(), nBins * sizeof(uint32_t), hipMemcpyHostToDevice));
 	RPWA_HIP_CHECK(hipMemcpy(b->d_acc, acc.data(), acc.size() * sizeof(double), hipMemcpyHostToDevice));
 	RPWA_HIP_CHECK(hipMemcpy(b->d_totAcc, totAcc.data(), nBins * sizeof(double), hipMemcpyHostToDevice));
-	// T = 0 (padding waves stay zero), flat^2 = 1 for every slot
+	RPWA_HIP_CHECK(hipMemset(b->d_noStatus, 0, F * sizeof(int32_t)));
+	// T = 0 (padding waves stay zero), flat^2 = 1 for every start
 	RPWA_HIP_CHECK(hipMemset(b->d_T32, 0, F * b->rowLen * sizeof(float)));
 	std::vector<double> ones(F, 1.);
 	RPWA_HIP_CHECK(hipMemcpy(b->d_flat2, ones.data(), F * sizeof(double), hipMemcpyHostToDevice));
-	std::vector<int32_t> active(nBins, 1);
-	RPWA_HIP_CHECK(hipMemcpy(b->d_binActive, active.data(), nBins * sizeof(int32_t), hipMemcpyHostToDevice));
 	b->finished = true;
 
-	// autotuner: event-kernel time at the drivers' start values for each
-	// number of workgroups per CU the kernel's occupancy allows
-	std::vector<double> start;
+	// autotuner: event-kernel time of an evaluation of every start at the
+	// drivers' start values, for each number of workgroups per CU the kernel's
+	// occupancy allows
+	std::vector<double> start, padded;
 	host_start_values(b, 1u, start);
-	std::vector<double> padded;
 	pad_params(b, start.data(), padded);
-	RPWA_HIP_CHECK(hipMemcpy(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice));
-	BatchPrepArgs pa{b->st.xt, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
-	hipLaunchKernelGGL(batch_prepare_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, pa);
-	RPWA_HIP_CHECK(hipGetLastError());
+	RPWA_TRY(prepare_eval(b, padded));
 	uint32_t best = 1;
 	if (b->wgPerCuFixed) {
 		best = std::min(b->wgPerCuFixed, b->maxWgPerCu);
 	} else if (b->maxWgPerCu > 1) {
 		double bestMs = 1e300;
 		for (uint32_t k = 1; k <= b->maxWgPerCu; ++k) {
-			RPWA_TRY(upload_plan(b, b->nCu * k));
-			RPWA_TRY(launch_event(b));  // warm-up
+			b->nWg = b->nCu * k;
+			for (int w = 0; w < 2; ++w)  // warm-up
+				RPWA_TRY(launch_event(b));
 			double ms = 0.;
 			RPWA_TRY(time_event(b, 3, &ms));
 			if (std::getenv("RPWA_VERBOSE"))
@@ -1254,7 +1286,7 @@ extern "C" int rpwa_batch_finish(rpwa_batch* b)
 		}
 	}
 	b->wgPerCu = best;
-	RPWA_TRY(upload_plan(b, b->nCu * best));
+	b->nWg = b->nCu * best;
 	RPWA_HIP_CHECK(hipStreamSynchronize(b->stream));
 	return RPWA_OK;
 }
@@ -1295,14 +1327,7 @@ extern "C" int rpwa_batch_eval(rpwa_batch* b, const double* par, double* value, 
 	const size_t F = nFits(b), P = b->P, nBins = b->bins.size();
 	std::vector<double> padded;
 	pad_params(b, par, padded);
-	RPWA_HIP_CHECK(hipMemcpyAsync(b->st.xt, padded.data(), padded.size() * sizeof(double), hipMemcpyHostToDevice,
-	                              b->stream));
-	std::vector<int32_t> active(nBins, 1);
-	RPWA_HIP_CHECK(hipMemcpyAsync(b->d_binActive, active.data(), nBins * sizeof(int32_t), hipMemcpyHostToDevice,
-	                              b->stream));
-	BatchPrepArgs pa{b->st.xt, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
-	hipLaunchKernelGGL(batch_prepare_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, pa);
-	RPWA_HIP_CHECK(hipGetLastError());
+	RPWA_TRY(prepare_eval(b, padded));
 	RPWA_TRY(launch_event(b));
 	RPWA_TRY(launch_reduce(b));
 	std::vector<double> ft(F), gt;
@@ -1367,9 +1392,9 @@ extern "C" int rpwa_batch_fit(rpwa_batch* b, const double* startPar, uint32_t se
 	BatchInitArgs ia{b->st, b->d_T32, b->d_flat2, b->W, b->P, b->rowLen, b->nStarts};
 	hipLaunchKernelGGL(batch_init_kernel, dim3((uint32_t)F), dim3(64), 0, b->stream, ia);
 	RPWA_HIP_CHECK(hipGetLastError());
-	RPWA_TRY(launch_status(b));
+	RPWA_TRY(launch_plan(b, false));
 
-	// one iteration = evaluate every running fit, reduce, minimiser step, bin flags
+	// one iteration = evaluate every running fit, reduce, minimiser step, next plan
 	if (!b->graph || b->graphMem != memory || b->graphMaxEval != maxEval || b->graphXtol != xtolRel ||
 	    b->graphFtol != ftolAbs) {
 		if (b->graph) {
@@ -1384,7 +1409,7 @@ extern "C" int rpwa_batch_fit(rpwa_batch* b, const double* startPar, uint32_t se
 		if (rc == RPWA_OK)
 			rc = launch_lbfgs(b, maxEval, xtolRel, ftolAbs);
 		if (rc == RPWA_OK)
-			rc = launch_status(b);
+			rc = launch_plan(b, false);
 		const hipError_t ec = hipStreamEndCapture(b->stream, &g);
 		if (rc != RPWA_OK)
 			return rc;
@@ -1448,10 +1473,9 @@ extern "C" int rpwa_batch_set_tile(rpwa_batch* b, uint32_t wgPerCu)
 	RPWA_TRY(check_batch(b, "rpwa_batch_set_tile", false));
 	b->wgPerCuFixed = wgPerCu;
 	if (b->finished) {
-		RPWA_HIP_CHECK(hipSetDevice(b->device));
 		const uint32_t k = wgPerCu ? std::min(wgPerCu, b->maxWgPerCu) : b->wgPerCu;
-		RPWA_TRY(upload_plan(b, b->nCu * k));
 		b->wgPerCu = k;
+		b->nWg = b->nCu * k;
 		if (b->graph) {
 			(void)hipGraphExecDestroy(b->graph);
 			b->graph = nullptr;
@@ -1470,7 +1494,7 @@ extern "C" int rpwa_batch_info(const rpwa_batch* b, uint64_t info[8])
 	}
 	info[0] = b->bins.size();
 	info[1] = b->nTiles;
-	info[2] = b->nSlots;
+	info[2] = b->nChunks;
 	info[3] = b->nWg;
 	info[4] = b->wgPerCu;
 	info[5] = b->Wp;
@@ -1488,9 +1512,8 @@ extern "C" int rpwa_batch_time_event_kernel(rpwa_batch* b, uint32_t reps, double
 		return RPWA_ERR_INVALID;
 	}
 	RPWA_HIP_CHECK(hipSetDevice(b->device));
-	std::vector<int32_t> active(b->bins.size(), 1);
-	RPWA_HIP_CHECK(hipMemcpy(b->d_binActive, active.data(), active.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-	for (int w = 0; w < 5; ++w)  // untimed: clocks and caches reach their steady state
+	RPWA_TRY(launch_plan(b, true));  // every start at the current trial points
+	for (int w = 0; w < 5; ++w)      // untimed: clocks and caches reach their steady state
 		RPWA_TRY(launch_event(b));
 	return time_event(b, reps, ms);
 }
