@@ -73,6 +73,12 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched-fit leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="profiling runs: only the batched leg, printed as its own JSON line")
+    ap.add_argument("--initializer", action="store_true",
+                    help="learned-initializer protocol only (training set of --init-train bins, best of 64 "
+                         "starts; evaluation on --init-test held-out bins), printed as its own JSON line")
+    ap.add_argument("--init-train", type=int, default=2000)
+    ap.add_argument("--init-test", type=int, default=200)
+    ap.add_argument("--init-epochs", type=int, default=400)
     ap.add_argument("--batch-fits", type=int, default=1,
                     help="timed batched fit runs (different seeds; the mean time is reported)")
     ap.add_argument("--batch-bins", type=int, default=48)
@@ -95,6 +101,9 @@ def main():
     log("device", dev)
     if args.only_batch:
         print(json.dumps({"batched": batched_leg(args, log)}), flush=True)
+        return
+    if args.initializer:
+        print(json.dumps({"initializer": initializer_leg(args, log)}), flush=True)
         return
 
     read_gbs, copy_gbs = capi.bandwidth_probe(4 << 30, 10)
@@ -263,6 +272,39 @@ def main():
     if cpu:
         out["vs_cpu"] = r_mix["rate"] / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def initializer_leg(args, log):
+    """Learned initializer: an MLP (3 x 512 GELU, FP32, trained by the library's
+    kernels) from per-bin data moments (88 diagonal + 256 off-diagonal Re/Im)
+    to start values; trained on best-of-64 fits of --init-train generated bins
+    (the batched workload's recipe), evaluated on --init-test held-out bins:
+    prediction + 7 random starts vs 64 random starts."""
+    from rootpwa_amd import initializer as I
+    from rootpwa_amd import synth
+    W = 88
+    seed_train, seed_test = SEED_BATCH + 100, SEED_BATCH + 200
+    train_sizes = synth.bin_sizes(seed_train, args.init_train)
+    test_sizes = synth.bin_sizes(seed_test, args.init_test)
+    t = time.perf_counter()
+    data = I.fitted_bins(seed_train, train_sizes, W, nStarts=64, fitSeed=11, log=log)
+    t_data = time.perf_counter() - t
+    init = I.Initializer(W, nPairs=256, nHidden=512, nHiddenLayers=3, seed=7)
+    t = time.perf_counter()
+    hist = init.train(data["moments"], data["best"], data["nEvents"], epochs=args.init_epochs, batchSize=64,
+                      lr=3e-4)
+    t_train = time.perf_counter() - t
+    log("initializer trained: %d bins, loss %.4f -> %.4f, validation %.4f (%.1f s)"
+        % (len(train_sizes), hist[0], hist[-1], init.validationLoss, t_train))
+    rep = I.evaluate(init, seed_test, test_sizes, W, nRandom=64, nMixed=8, fitSeed=13, log=log)
+    per_bin = rep.pop("per_bin")
+    rep.update({"train_bins": len(train_sizes), "training_data_s": t_data, "training_s": t_train,
+                "epochs": args.init_epochs, "validation_loss": float(init.validationLoss),
+                "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+                "network": "775 (600 moments + 175 spectral) -> 3 x 512 GELU -> 176 (correction to the spectral estimate), FP32, Adam lr 3e-4, batch 64",
+                "seeds": {"train": seed_train, "test": seed_test}})
+    log("initializer evaluation:", rep)
+    return rep
 
 
 def batched_leg(args, log):

@@ -181,6 +181,13 @@ int rpwa_integral(int device, const double* amps, uint64_t nmbEvents, uint32_t n
 /* amplitudes (and weights) already in device memory */
 int rpwa_integral_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
                          const double* d_weights, double* outSums);
+/* Data moments for the learned initializer (no counterpart in the reference):
+ * with a_i(e) = A_i(e) / sqrt(normDiag_i) (normDiag: the normalisation
+ * integral's diagonal elements), outMoments [W][W] complex =
+ * mean_e a_i(e) conj(a_j(e)), or with perEventNorm mean_e a_i conj(a_j) / |a(e)|^2,
+ * which removes any overall per-event scale.  FP64 MFMA, d_amps [N][W] complex128. */
+int rpwa_data_moments_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                             const double* normDiag, int perEventNorm, double* outMoments);
 
 /* ---------------------------------------------------------------------------
  * Synthetic amplitudes (the generator behind the benchmark workloads).

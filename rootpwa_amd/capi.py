@@ -66,6 +66,7 @@ PROTOTYPES = {
     "rpwa_bandwidth_probe": (_i, [_i, _u64, _i, _dp, _dp]),
     "rpwa_integral": (_i, [_i, _dp, _u64, _u32, _dp, _dp]),
     "rpwa_integral_device": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
+    "rpwa_data_moments_device": (_i, [_i, _vp, _u64, _u32, _dp, _i, _dp]),
     "rpwa_gen_wave_scales": (_i, [_u64, _u32, _dp]),
     "rpwa_gen_prod_amps": (_i, [_u64, _u32, _u32, _u32p, _dp]),
     "rpwa_gen_amps_device": (_i, [_i, _u64, _u32, _u64, _u64, _u32, _dp, _vp]),
@@ -315,6 +316,16 @@ def integral_device(d_amps, M, W, d_weights=None, device=0):
     out = np.zeros((W, W), dtype=np.complex128)
     call("rpwa_integral_device", int(device), _vp(d_amps), int(M), int(W),
          _vp(d_weights) if d_weights else None, _cd(out))
+    return out
+
+
+def data_moments_device(d_amps, N, W, normDiag, perEventNorm=True, device=0):
+    """[W][W] complex mean moments of amplitudes normalised by sqrt(normDiag) (per-event
+    normalised with perEventNorm), FP64 MFMA on the device."""
+    nd = np.ascontiguousarray(normDiag, dtype=np.float64)
+    out = np.zeros((W, W), dtype=np.complex128)
+    call("rpwa_data_moments_device", int(device), _vp(d_amps), int(N), int(W), _dptr(nd), int(bool(perEventNorm)),
+         _cd(out))
     return out
 
 

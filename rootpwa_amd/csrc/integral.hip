@@ -17,6 +17,7 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -427,7 +428,86 @@ static int integral_run(int device, const double* d_amps, uint64_t M, uint32_t W
 	return RPWA_OK;
 }
 
+// per event w_e = 1 / sum_i |A_i(e)|^2 / I_ii (0 for an all-zero event)
+__global__ void inverse_norm_kernel(const double2* __restrict__ amps, uint64_t M, uint32_t W,
+                                    const double* __restrict__ invDiag, double* __restrict__ w)
+{
+	const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= M)
+		return;
+	double s = 0.;
+	for (uint32_t i = 0; i < W; ++i) {
+		const double2 a = amps[e * W + i];
+		s += (a.x * a.x + a.y * a.y) * invDiag[i];
+	}
+	w[e] = s > 0. ? 1. / s : 0.;
+}
+
+static int moments_run(int device, const double* d_amps, uint64_t M, uint32_t W, const double* normDiag,
+                       int perEventNorm, double* out)
+{
+	std::vector<double> C;
+	double wtot = 0.;
+	const uint32_t D = 2 * W;
+	double* d_w = nullptr;
+	double* d_inv = nullptr;
+	std::vector<double> inv(W);
+	for (uint32_t i = 0; i < W; ++i)
+		inv[i] = normDiag[i] > 0. ? 1. / normDiag[i] : 0.;
+	if (perEventNorm) {
+		RPWA_HIP_CHECK(hipSetDevice(device));
+		RPWA_HIP_CHECK(hipMalloc(&d_w, M * sizeof(double)));
+		if (hipMalloc(&d_inv, W * sizeof(double)) != hipSuccess ||
+		    hipMemcpy(d_inv, inv.data(), W * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+			(void)hipFree(d_w);
+			if (d_inv)
+				(void)hipFree(d_inv);
+			set_error("rpwa_data_moments_device: allocation or copy failed");
+			return RPWA_ERR_HIP;
+		}
+		hipLaunchKernelGGL(inverse_norm_kernel, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, 0,
+		                   reinterpret_cast<const double2*>(d_amps), M, W, d_inv, d_w);
+		const hipError_t e = hipDeviceSynchronize();
+		(void)hipFree(d_inv);
+		if (e != hipSuccess) {
+			(void)hipFree(d_w);
+			set_error(std::string("rpwa_data_moments_device: ") + hipGetErrorString(e));
+			return RPWA_ERR_HIP;
+		}
+	}
+	const int st = syrk_run(device, d_amps, M, D, d_w, true, nullptr, nullptr, C, &wtot);
+	if (d_w)
+		(void)hipFree(d_w);
+	if (st != RPWA_OK)
+		return st;
+	for (uint32_t i = 0; i < W; ++i)
+		for (uint32_t j = 0; j < W; ++j) {
+			const double re = C[(size_t)(2 * i) * D + 2 * j] + C[(size_t)(2 * i + 1) * D + 2 * j + 1];
+			const double im = C[(size_t)(2 * i + 1) * D + 2 * j] - C[(size_t)(2 * i) * D + 2 * j + 1];
+			const double sc = std::sqrt(inv[i] * inv[j]) / (double)M;
+			out[2 * ((size_t)i * W + j)] = re * sc;
+			out[2 * ((size_t)i * W + j) + 1] = im * sc;
+		}
+	return RPWA_OK;
+}
+
 }  // namespace rpwa_amd
+
+extern "C" int rpwa_data_moments_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                                        const double* normDiag, int perEventNorm, double* outMoments)
+{
+	rpwa_amd::clear_error();
+	if (!d_amps || !normDiag || !outMoments || nmbEvents == 0 || nmbWaves == 0) {
+		rpwa_amd::set_error("rpwa_data_moments_device: null pointer or empty input");
+		return RPWA_ERR_INVALID;
+	}
+	try {
+		return rpwa_amd::moments_run(device, d_amps, nmbEvents, nmbWaves, normDiag, perEventNorm, outMoments);
+	} catch (const std::exception& ex) {
+		rpwa_amd::set_error(ex.what());
+		return RPWA_ERR_NOMEM;
+	}
+}
 
 extern "C" int rpwa_integral(int device, const double* amps, uint64_t nmbEvents, uint32_t nmbWaves,
                              const double* weights, double* outSums)

@@ -12,7 +12,8 @@ from oracle import oracle as O
 from tests import helpers as H
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", f) for f in ("rootpwa_amd.h", "rootpwa_amd_pwalik.h", "rootpwa_amd_batch.h")]
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("rootpwa_amd.h", "rootpwa_amd_pwalik.h", "rootpwa_amd_batch.h",
+                                                           "rootpwa_amd_mlp.h")]
 
 
 def declared_functions(path):
@@ -31,9 +32,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_bindings_cover_the_headers():
-    from rootpwa_amd import batch, capi, likelihood
+    from rootpwa_amd import batch, capi, likelihood, mlp
     names = [n for h in HEADERS for n in declared_functions(h)]
-    bound = set(capi.PROTOTYPES) | set(likelihood.PROTOTYPES) | set(batch.PROTOTYPES)
+    bound = set(capi.PROTOTYPES) | set(likelihood.PROTOTYPES) | set(batch.PROTOTYPES) | set(mlp.PROTOTYPES)
     assert [n for n in names if n not in bound] == []
 
 

@@ -300,3 +300,62 @@ def test_generator_restatement():
     sig = np.abs(data @ prod[0, 1]) ** 2
     pool_a = O.gen_amps(seed, 4, 0, 20000, np.ones(4))
     assert np.mean(sig) > 1.5 * np.mean(np.abs(pool_a @ prod[0, 1]) ** 2)
+
+
+# ---- the learned initializer's network (oracle/mlp_oracle.py) ---------------------------------
+
+def test_mlp_oracle_gelu_closed_forms():
+    from oracle import mlp_oracle as M
+    import math
+    z = np.array([-30.0, -1.0, 0.0, 0.5, 1.0, 30.0])
+    g = M.gelu(z)
+    # GELU(z) = z Phi(z): Phi(0) = 1/2, Phi(1) = 0.841344746068543, tails 0 and 1
+    assert g[2] == 0.0 and abs(g[0]) < 1e-150 and g[5] == 30.0
+    assert g[4] == pytest.approx(0.841344746068542948585, rel=1e-15)
+    assert g[1] == pytest.approx(-1.0 * (1.0 - 0.841344746068542948585), rel=1e-14)
+    # derivative: Phi(z) + z phi(z); at 0 it is 1/2, and it matches central differences
+    assert M.gelu_grad(np.array([0.0]))[0] == 0.5
+    h = 1e-6
+    zz = np.linspace(-4, 4, 33)
+    fd = (M.gelu(zz + h) - M.gelu(zz - h)) / (2 * h)
+    assert np.max(np.abs(fd - M.gelu_grad(zz))) < 1e-9
+    assert math.isclose(float(M.gelu(np.array([0.5]))[0]), 0.5 * 0.691462461274013103637, rel_tol=1e-15)
+
+
+def test_mlp_oracle_gradient_against_finite_differences():
+    """The hand-written backward pass against central differences of the loss, for every parameter
+    of a small network (a dropped term, a transposed product or a wrong activation derivative fails)."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(11)
+    shape = (5, 7, 2, 3)
+    n = sum(i * o + o for i, o in M.shapes(*shape))
+    p = rng.normal(size=n) * 0.7
+    x = rng.normal(size=(6, 5))
+    t = rng.normal(size=(6, 3))
+    layers = M.unpack(p, *shape)
+    loss, g = M.loss_grad(layers, x, t)
+    assert loss == pytest.approx(M.loss_only(layers, x, t), rel=1e-15)
+    h = 1e-6
+    fd = np.zeros(n)
+    for k in range(n):
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd[k] = (M.loss_only(M.unpack(pp, *shape), x, t) - M.loss_only(M.unpack(pm, *shape), x, t)) / (2 * h)
+    assert np.max(np.abs(fd - g)) / np.max(np.abs(g)) < 1e-8
+
+
+def test_mlp_oracle_linear_limit():
+    """With no hidden layer the network is y = x W + b and the MSE gradient is the normal-equation residual."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(9, 4))
+    t = rng.normal(size=(9, 2))
+    p = rng.normal(size=4 * 2 + 2)
+    layers = M.unpack(p, 4, 1, 0, 2)
+    W, b = layers[0]
+    loss, g = M.loss_grad(layers, x, t)
+    r = x @ W + b - t
+    assert loss == pytest.approx(np.sum(r ** 2) / r.size, rel=1e-14)
+    assert np.allclose(g[:8].reshape(4, 2), 2 * x.T @ r / r.size, rtol=1e-13)
+    assert np.allclose(g[8:], 2 * r.sum(axis=0) / r.size, rtol=1e-13)
