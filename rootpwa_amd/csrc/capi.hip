@@ -120,7 +120,9 @@ int configure_grid(rpwa_lik_ctx* c, int rank)
 		return st;
 	}
 	c->nWg[rank] = c->nCu * std::min(wgPerCu, 8);
-	const size_t need = (size_t)c->nWg[rank] * partial_elems(kMaxRank, c->lay.wmax);
+	// sized once for the largest grid any tile can get (8 workgroups per CU), so a
+	// tile change never moves the buffer under a kernel argument that still points at it
+	const size_t need = (size_t)c->nCu * 8 * partial_elems(kMaxRank, c->lay.wmax);
 	if (need > c->partialCap) {
 		hipFree(c->d_partials);
 		c->d_partials = nullptr;
@@ -238,6 +240,7 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			c->tile[rank] = ts;
 			if (int st = configure_grid(c, rank))
 				return st;
+			a.partials = c->d_partials;  // the grid configuration owns the buffer
 			a.lanes0 = ts.lanes0;
 			a.lanes1 = ts.lanes1;
 			std::vector<float> ms;
