@@ -350,13 +350,22 @@ def batched_leg(args, log):
     log("batched fits: %.3g fits/hr (%.3f s for %d fits, %d iterations, %.1f evaluations per fit)"
         % (out["fits_per_hr"], el, nfits, info["lastEvaluations"], ne.mean()))
     if not args.skip_oracle:
-        # the batch's lnL and gradient against the oracle at the best fit of the
-        # smallest and the largest bin (production amplitudes rounded to FP32, as
-        # the batch contracts them; data regenerated from the same counters)
+        # the batch's lnL and gradient against the oracle in the smallest and the
+        # largest bin, at the timed fit's start point of the best start and at its
+        # minimum (production amplitudes rounded to FP32, as the batch contracts
+        # them; data regenerated from the same counters).  The gradient error is
+        # relative to the gradient's size at the start: at the minimum the
+        # gradient itself vanishes.
         from oracle import oracle as O
-        pts = par.copy()
-        pts[..., :-1] = pts[..., :-1].astype(np.float32).astype(np.float64)
+
+        def fp32_pars(x):
+            x = x.copy()
+            x[..., :-1] = x[..., :-1].astype(np.float32).astype(np.float64)
+            return x
+        pts = fp32_pars(par)
+        p0 = fp32_pars(bf.startValues(2 + runs - 1))
         v, g = bf.eval(pts)
+        v0, g0 = bf.eval(p0)
         checks = []
         for b in (int(np.argmin(sizes)), int(np.argmax(sizes))):
             n = int(sizes[b])
@@ -369,9 +378,16 @@ def batched_leg(args, log):
             Lo.finish_init()
             del data
             s_best = int(np.argmin(val[b]))
+            f0_ref, g0_ref = Lo.fdf(p0[b, s_best], threads=args.oracle_threads)
             f_ref, g_ref = Lo.fdf(pts[b, s_best], threads=args.oracle_threads)
-            checks.append({"bin": b, "events": n, "dlnL": abs(v[b, s_best] - f_ref),
-                           "grad_rel": float(np.max(np.abs(g[b, s_best] - g_ref)) / np.max(np.abs(g_ref)))})
+            gscale = float(np.max(np.abs(g0_ref)))
+            checks.append({"bin": b, "events": n,
+                           "start": {"dlnL": abs(v0[b, s_best] - f0_ref),
+                                     "grad_rel": float(np.max(np.abs(g0[b, s_best] - g0_ref))) / gscale},
+                           "minimum": {"dlnL": abs(v[b, s_best] - f_ref),
+                                       "grad_err_over_start_scale":
+                                           float(np.max(np.abs(g[b, s_best] - g_ref))) / gscale,
+                                       "grad_max_abs": float(np.max(np.abs(g_ref)))}})
             del Lo
         out["check"] = checks
         log("batched oracle check:", checks)
