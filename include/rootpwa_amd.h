@@ -129,16 +129,20 @@ int rpwa_lik_kernel_times(rpwa_lik_ctx* ctx, double* eventKernel, double* reduce
 
 /* Shape and launch configuration of a context:
  * info[0] events, [1] W(refl -1), [2] W(refl +1), [3] precision, [4] lanes per event,
- * [5] slots per lane, [6] workgroups, [7] threads per
+ * [5] slots per lane (events per tile for the staged kernel), [6] workgroups, [7] threads per
  * workgroup, [8] bytes of amplitudes; [4]-[6] for the lowest rank evaluated so far. */
 int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
 
 /* Event tiles: before the first evaluation of a given rank the context times
- * every feasible tile (lanes per event in {8,16,32,64}) on its own
- * amplitudes and keeps the
- * fastest for that rank; results of a rank always come from one tile.
- * rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the environment) keeps
- * the modelled default instead; rpwa_lik_set_tile fixes the tile of every rank. */
+ * every feasible kernel variant on its own amplitudes and keeps the fastest
+ * for that rank; results of a rank always come from one variant.  Variants:
+ * lanes per event in {8,16,32,64} (register tiles: a group of lanes holds one
+ * event's amplitude row and the production amplitudes of its columns) and
+ * lanes per event 1 (LDS-staged: tiles of 16..64 events pass through LDS, one
+ * lane contracts one event, production amplitudes come through the scalar
+ * unit).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
+ * environment) keeps the modelled default instead; rpwa_lik_set_tile fixes the
+ * variant of every rank (RPWA_ERR_UNSUPPORTED if the row does not fit it). */
 int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable);
 int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
 

@@ -62,6 +62,36 @@ bool make_tile(const AmpLayout& lay, int lanes, TileShape& ts)
 	return false;
 }
 
+// Tile of the LDS-staged kernel: the largest feasible number of events per tile.
+bool make_stage_tile(const AmpLayout& lay, int rank, TileShape& ts)
+{
+	for (int e : {64, 32, 16}) {
+		StagePlan p;
+		if (stage_plan(lay, e, rank, p)) {
+			ts = TileShape();
+			ts.lanes = 1;
+			ts.slots = e;
+			ts.events = e;
+			return true;
+		}
+	}
+	return false;
+}
+
+// T of every amplitude-row column for the staged kernel: [rowLen][ranks],
+// zero on the padding columns and for ranks beyond `rank`.
+void fill_tpad(const AmpLayout& lay, const double2* prod, int rank, int ranks, double2* tpad)
+{
+	for (uint32_t col = 0; col < lay.rowLen; ++col) {
+		const int s = col >= lay.secOff[1] ? 1 : 0;
+		const uint32_t i = col - lay.secOff[s];
+		const bool valid = i < lay.nW[s];
+		for (int r = 0; r < ranks; ++r)
+			tpad[(size_t)col * ranks + r] =
+			    valid && r < rank ? prod[((size_t)r * 2 + s) * lay.wmax + i] : make_double2(0., 0.);
+	}
+}
+
 // Modelled default tile (used when autotuning is off): VALU instructions per
 // event of the complex MACs of the group's slots, the per-event log/division
 // and the butterflies, rank 2 assumed.
@@ -162,7 +192,8 @@ int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
 
 int alloc_eval_buffers(rpwa_lik_ctx* c)
 {
-	const size_t nProd = (size_t)kMaxRank * 2 * c->lay.wmax;
+	// production amplitudes [rank][2][wmax], followed by the staged kernel's T per row column
+	const size_t nProd = (size_t)kMaxRank * 2 * c->lay.wmax + (size_t)c->lay.rowLen * kMaxRank;
 	const size_t nOut = partial_elems(kMaxRank, c->lay.wmax);
 	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
 	RPWA_HIP_CHECK(hipMalloc(&c->d_out, nOut * sizeof(double)));
@@ -209,10 +240,12 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 {
 	const uint32_t wmax = c->lay.wmax;
 	const size_t nProd = (size_t)rank * 2 * wmax;
-	std::vector<double2> prod(nProd);
+	const int ranks = stage_rank_cap(rank);
+	std::vector<double2> prod(nProd + (size_t)c->lay.rowLen * ranks);
 	for (size_t i = 0; i < nProd; ++i)
 		prod[i] = make_double2(0.5 + 0.01 * (double)(i % 7), 0.25 - 0.01 * (double)(i % 5));
-	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, prod.data(), nProd * sizeof(double2), hipMemcpyHostToDevice,
+	fill_tpad(c->lay, prod.data(), rank, ranks, prod.data() + nProd);
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, prod.data(), prod.size() * sizeof(double2), hipMemcpyHostToDevice,
 	                              c->stream));
 	const uint32_t nElem = partial_elems(rank, wmax);
 	EventArgs a;
@@ -226,6 +259,7 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	a.wmax = wmax;
 	a.rank = rank;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
+	a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nProd;
 	a.flat2 = 0.01;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
@@ -233,9 +267,9 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	TileShape best = keep;
 	double bestT = 1e300;
 	const int reps = 5;
-	for (int lanes : {8, 16, 32, 64}) {
+	for (int lanes : {8, 16, 32, 64, 1}) {
 			TileShape ts;
-			if (!make_tile(c->lay, lanes, ts))
+			if (lanes == 1 ? !make_stage_tile(c->lay, rank, ts) : !make_tile(c->lay, lanes, ts))
 				continue;
 			c->tile[rank] = ts;
 			if (int st = configure_grid(c, rank))
@@ -258,8 +292,8 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			const double med = ms[ms.size() / 2];
 			if (std::getenv("RPWA_VERBOSE"))
 				std::fprintf(stderr, "[rpwa autotune] precision %d rank %d lanes %d slots %d (%d+%d lanes) "
-				             "workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots, ts.lanes0,
-				             ts.lanes1, c->nWg[rank], med);
+				             "events/tile %d workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots,
+				             ts.lanes0, ts.lanes1, ts.events, c->nWg[rank], med);
 			if (med < bestT) {
 				bestT = med;
 				best = ts;
@@ -302,8 +336,15 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 		c->tileFixed[rank] = true;
 	}
 	std::memcpy(c->h_prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2));
-	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, (size_t)nmbProdAmps * sizeof(double2),
-	                              hipMemcpyHostToDevice, c->stream));
+	size_t nUp = nmbProdAmps;
+	if (c->tile[rank].events > 0) {
+		const int ranks = stage_rank_cap((int)rank);
+		fill_tpad(c->lay, reinterpret_cast<const double2*>(c->h_prod), (int)rank, ranks,
+		          reinterpret_cast<double2*>(c->h_prod) + nmbProdAmps);
+		nUp += (size_t)c->lay.rowLen * ranks;
+	}
+	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, nUp * sizeof(double2), hipMemcpyHostToDevice,
+	                              c->stream));
 	const uint32_t nElem = partial_elems((int)rank, wmax);
 	EventArgs a;
 	a.amps = c->d_amps;
@@ -318,6 +359,7 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.lanes0 = c->tile[rank].lanes0;
 	a.lanes1 = c->tile[rank].lanes1;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
+	a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps;
 	a.flat2 = flat * flat;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
@@ -671,19 +713,23 @@ extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
 		return RPWA_ERR_STATE;
 	}
 	const uint32_t lanes = lanesPerEvent;
-	if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
-		set_error("rpwa_lik_set_tile: lanes per event must be 8, 16, 32 or 64");
+	if (lanes != 1 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
+		set_error("rpwa_lik_set_tile: lanes per event must be 1 (LDS-staged), 8, 16, 32 or 64");
 		return RPWA_ERR_INVALID;
 	}
 	TileShape ts;
-	if (!make_tile(ctx->lay, (int)lanes, ts)) {
-		set_error("rpwa_lik_set_tile: the amplitude row does not fit " + std::to_string(lanes) +
-		          " lanes x 8 slots");
+	if (lanes == 1 ? !make_stage_tile(ctx->lay, kMaxRank, ts) : !make_tile(ctx->lay, (int)lanes, ts)) {
+		set_error(lanes == 1 ? std::string("rpwa_lik_set_tile: the amplitude row does not fit the LDS-staged "
+		                                   "kernel (at most 256 16-byte units per row)")
+		                     : "rpwa_lik_set_tile: the amplitude row does not fit " + std::to_string(lanes) +
+		                           " lanes x 8 slots");
 		return RPWA_ERR_UNSUPPORTED;
 	}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
 	for (int r = 1; r <= kMaxRank; ++r) {
+		if (lanes == 1)
+			make_stage_tile(ctx->lay, r, ts);
 		ctx->tile[r] = ts;
 		ctx->tileFixed[r] = true;
 		if (int st = configure_grid(ctx, r))

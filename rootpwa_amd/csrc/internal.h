@@ -51,6 +51,21 @@ struct TileShape {
 	int slots = 1;
 	int lanes0 = 0;
 	int lanes1 = 0;
+	int events = 0;  // > 0: the LDS-staged kernel (one lane per event) with tiles of this many events
+};
+
+// Geometry of the LDS-staged kernel (stage_kernel.hip) for one layout, tile size and rank.
+struct StagePlan {
+	int events = 0;
+	int ranks = 1;           // ranks held by the kernel instantiation (1, 2 or 4)
+	uint32_t rowUnits = 0;   // 16-byte units per amplitude row
+	uint32_t rowUnitsP = 0;  // LDS row stride (odd)
+	uint32_t loads = 0;      // 16-byte loads per thread and tile
+	uint32_t gGroups0 = 0;   // unit groups of the refl -1 sector in the gradient phase
+	uint32_t gGroups = 0;    // unit groups of both sectors
+	uint32_t gSubs = 0;      // event subsets per unit group in the gradient phase
+	uint32_t stageBytes = 0;
+	size_t ldsBytes = 0;
 };
 
 struct EventArgs {
@@ -63,6 +78,7 @@ struct EventArgs {
 	int rank;
 	int lanes0, lanes1;
 	const double2* prod;   // device [rank][2][wmax] complex
+	const double2* tpad;   // device [rowLen][ranks of the staged kernel]: T per row column, 0 on padding
 	double flat2;
 	double* partials;      // [nWg][nElem]
 	uint32_t nElem;
@@ -76,6 +92,10 @@ inline uint32_t partial_elems(int rank, uint32_t wmax) { return 3u + 2u * (uint3
 hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
                                bool grad, int nWg, hipStream_t s);
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
+bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p);
+int stage_rank_cap(int rank);
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s);
+int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu);
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s);
 // weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T

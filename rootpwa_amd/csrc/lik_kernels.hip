@@ -164,6 +164,8 @@ size_t lds_bytes(const AmpLayout& lay, const TileShape& ts, int rank)
 hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
                                bool /*grad*/, int nWg, hipStream_t s)
 {
+	if (ts.events > 0)
+		return launch_stage_kernel(lay, ts.events, a, nWg, s);
 	EventKernel k = pick_kernel(lay, ts, a.rank);
 	hipLaunchKernelGGL(k, dim3(nWg), dim3(kWgThreads), lds_bytes(lay, ts, a.rank), s, a);
 	return hipGetLastError();
@@ -171,6 +173,8 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
 
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu)
 {
+	if (ts.events > 0)
+		return stage_kernel_occupancy(lay, ts.events, rank, wgPerCu);
 	EventKernel k = pick_kernel(lay, ts, rank);
 	int n = 0;
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kWgThreads, lds_bytes(lay, ts, rank));

@@ -1,0 +1,483 @@
+// stage_kernel.hip -- the LDS-staged event kernel: one lane per event.
+//
+// Computes exactly what the register-tile kernels of lik_kernels.hip compute
+// (sum_e -ln sigma_e compensated, sum_e -2/sigma_e and the event part of the
+// gradient, pwaLikelihood.cc:213-244), with a different work decomposition
+// that keeps neither the production amplitudes nor a slice of the amplitude
+// row in the registers of every lane:
+//
+//   * a tile of E consecutive events is one contiguous block of the amplitude
+//     array; the 256 threads of a workgroup fetch it with coalesced 16-byte
+//     loads one tile ahead (the loads of tile t+1 are in flight while tile t is
+//     computed) and store it into LDS with an odd row stride in 16-byte units,
+//     so that lanes reading the same column of consecutive rows hit distinct
+//     banks;
+//   * S phase: wave w contracts column slice w of every row, lane = event.
+//     The production amplitudes of a column are the same for every lane and
+//     are read through the scalar unit; waves 1..3 leave their partial S in
+//     LDS and wave 0 adds them in the order 0,1,2,3;
+//   * sigma phase (wave 0, lane = event): sigma, -ln sigma into a per-lane
+//     Kahan sum, w = -2/sigma, and u = w S stored in LDS;
+//   * G phase: thread = (16-byte column unit, event subset); it adds
+//     u conj(A) of its unit over its events of the tile in FP64 and keeps the
+//     sums across tiles;
+//   * end: the event subsets are added in fixed order and written as the
+//     workgroup's partial row, in the format of the register-tile kernels, so
+//     event_reduce and everything after it is shared.
+// Every sum has a fixed order for a fixed grid: results are bitwise
+// reproducible and invariant under T -> -T of a (rank, sector) block.
+#include "event_kernel.h"
+
+namespace rpwa_amd {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxLoads = 12;  // 16-byte loads per thread and tile (E * rowUnits <= 3072)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // one 16-byte unit
+
+// G phase: consecutive units of one sector handled by one thread (the
+// weights u of an event are read once for all of them)
+template <bool MIXED, int R>
+constexpr int group_units() { return (MIXED ? 2 : 1) * R <= 2 ? 4 : 2; }
+// LDS exchange area: complex values per event (2 sectors x R ranks, plus one
+// so that lanes reading consecutive events hit distinct banks)
+template <int R>
+constexpr int part_stride() { return 2 * R + 1; }
+typedef const __attribute__((address_space(4))) double cdouble;
+
+// T of one row column through the scalar unit (the index is the same in every lane)
+__device__ __forceinline__ double2 scalar_t(cdouble* tp, uint32_t i)
+{
+	return make_double2(tp[2 * i], tp[2 * i + 1]);
+}
+
+struct StageArgs {
+	const f32x4* amps;      // nEvents rows of rowUnits 16-byte units
+	uint64_t nEvents;
+	uint64_t lastUnit;      // nEvents * rowUnits - 1
+	uint32_t nTiles;
+	uint32_t rowUnits;      // 16-byte units per row
+	uint32_t rowUnitsP;     // LDS row stride in units (odd)
+	uint32_t tileUnits;     // E * rowUnits
+	uint32_t loads;         // ceil(tileUnits / 256)
+	uint32_t sliceUnits;    // units per wave in the S phase
+	uint32_t secUnit;       // first unit of the refl +1 sector
+	uint32_t gGroups0;      // G-phase unit groups of the refl -1 sector
+	uint32_t gGroups;       // G-phase unit groups of both sectors (<= 256)
+	uint32_t gSubs;         // event subsets per unit group in the G phase
+	uint32_t stageBytes;    // (E * rowUnitsP + 1) * 16: the rows and the dump unit
+	uint32_t secOff1, nW0, nW1, wmax;
+	const double2* tpad;    // [rowUnits * P][R]: T of each row column, zero on padding
+	double flat2;
+	double* partials;       // [nWg][nElem]
+	uint32_t nElem;
+};
+
+// all LDS writes of this wave done, then the workgroup barrier (no wait for
+// the global loads in flight)
+__device__ __forceinline__ void lds_barrier()
+{
+	asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double2 unit_cplx(const f32x4 v, int k)
+{
+	if (MIXED)
+		return k == 0 ? make_double2((double)v.x, (double)v.y) : make_double2((double)v.z, (double)v.w);
+	return make_double2(__hiloint2double(__float_as_int(v.y), __float_as_int(v.x)),
+	                    __hiloint2double(__float_as_int(v.w), __float_as_int(v.z)));
+}
+
+// The three phases of one staged tile: S (all waves), sigma (wave 0), G (all
+// threads).  The stage holds the tile; part is the LDS exchange area.
+template <bool MIXED, int R, int E>
+__device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* stage, double2* part, int lane,
+                                            int wid, uint32_t tile, bool gOn, uint32_t gU0, uint32_t gEnd,
+                                            uint32_t gSub, int gSec,
+                                            double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R],
+                                            double& lnS, double& lnC, double& wsum)
+{
+	constexpr int P = MIXED ? 2 : 1;
+	constexpr int UG = group_units<MIXED, R>();
+	constexpr int PS = part_stride<R>();
+	// ---- S phase: wave wid, slice of units [u0, u1), lane = event ----
+	double2 S[2][R];
+#pragma unroll
+	for (int s = 0; s < 2; ++s)
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			S[s][r] = make_double2(0., 0.);
+	if (E == 64 || lane < E) {
+		const f32x4* row = stage + (uint32_t)lane * a.rowUnitsP;
+		const uint32_t u0 = (uint32_t)wid * a.sliceUnits;
+		const uint32_t u1 = min(u0 + a.sliceUnits, a.rowUnits);
+		const uint32_t um = min(max(a.secUnit, u0), u1);
+		cdouble* tp = (cdouble*)(const double*)a.tpad;
+#pragma unroll 4
+		for (uint32_t u = u0; u < um; ++u) {
+			const f32x4 v = row[u];
+#pragma unroll
+			for (int k = 0; k < P; ++k) {
+				const double2 x = unit_cplx<MIXED>(v, k);
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					ek::cmac(S[0][r], scalar_t(tp, (u * P + k) * R + r), x);
+			}
+		}
+#pragma unroll 4
+		for (uint32_t u = um; u < u1; ++u) {
+			const f32x4 v = row[u];
+#pragma unroll
+			for (int k = 0; k < P; ++k) {
+				const double2 x = unit_cplx<MIXED>(v, k);
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					ek::cmac(S[1][r], scalar_t(tp, (u * P + k) * R + r), x);
+			}
+		}
+		if (wid > 0) {
+			double2* p = part + ((size_t)(wid - 1) * E + lane) * PS;
+#pragma unroll
+			for (int s = 0; s < 2; ++s)
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					p[s * R + r] = S[s][r];
+		}
+	}
+	lds_barrier();
+
+	// ---- sigma phase: wave 0, lane = event ----
+	if (wid == 0 && (E == 64 || lane < E)) {
+#pragma unroll
+		for (int w = 1; w < kWaves; ++w) {
+			const double2* p = part + ((size_t)(w - 1) * E + lane) * PS;
+#pragma unroll
+			for (int s = 0; s < 2; ++s)
+#pragma unroll
+				for (int r = 0; r < R; ++r) {
+					const double2 q = p[s * R + r];
+					S[s][r].x += q.x;
+					S[s][r].y += q.y;
+				}
+		}
+		double sigma = 0.;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			sigma += S[0][r].x * S[0][r].x + S[0][r].y * S[0][r].y;
+			sigma += S[1][r].x * S[1][r].x + S[1][r].y * S[1][r].y;
+		}
+		sigma += a.flat2;
+		double w = 0.;
+		if ((uint64_t)tile * E + lane < a.nEvents) {
+			ek::kahan_add(lnS, lnC, -log(sigma));
+			w = -2. / sigma;
+			wsum += w;
+		}
+		double2* p = part + (size_t)lane * PS;  // u replaces wave 1's partial
+#pragma unroll
+		for (int s = 0; s < 2; ++s)
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				p[s * R + r] = make_double2(w * S[s][r].x, w * S[s][r].y);
+	}
+	lds_barrier();
+
+	// ---- G phase: thread = (group of UG units of one sector, event subset) ----
+	if (gOn) {
+		const double2* up = part + gSec * R;
+		uint32_t cu[UG];  // unit of each group slot (clamped into the row; slots past the sector are discarded)
+#pragma unroll
+		for (int j = 0; j < UG; ++j)
+			cu[j] = min(gU0 + (uint32_t)j, a.rowUnits - 1);
+		for (uint32_t e = gSub; e < (uint32_t)E; e += a.gSubs) {
+			const f32x4* row = stage + e * a.rowUnitsP;
+			double2 u[R];
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				u[r] = up[e * PS + r];
+#pragma unroll
+			for (int j = 0; j < UG; ++j) {
+				const f32x4 v = row[cu[j]];
+#pragma unroll
+				for (int k = 0; k < P; ++k) {
+					const double2 x = unit_cplx<MIXED>(v, k);
+#pragma unroll
+					for (int r = 0; r < R; ++r)
+						ek::cmac_conj(G[j][k][r], u[r], x);
+				}
+			}
+		}
+	}
+	(void)gEnd;
+}
+
+template <bool MIXED, int R, int E>
+__global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
+{
+	constexpr int P = MIXED ? 2 : 1;  // complex values per 16-byte unit
+	constexpr int PS = part_stride<R>();
+	extern __shared__ f32x4 smem[];
+	f32x4* stage = smem;  // [E][rowUnitsP] units, then the dump unit
+	double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(smem) + a.stageBytes);  // [3][E][PS]
+
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+	// LDS positions of this thread's loads (the same for every tile); a load
+	// past the tile goes to the dump unit that follows the stage
+	const uint32_t dumpUnit = (uint32_t)E * a.rowUnitsP;
+	uint32_t ldsPos[kMaxLoads];
+	uint32_t ldOff[kMaxLoads];
+#pragma unroll
+	for (int k = 0; k < kMaxLoads; ++k) {
+		const uint32_t g = (uint32_t)tid + (uint32_t)kThreads * k;
+		const uint32_t row = g / a.rowUnits;
+		const bool in = g < a.tileUnits;
+		ldsPos[k] = in ? row * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
+		ldOff[k] = in ? g : (uint32_t)tid;
+	}
+
+	// G-phase role: unit group gi of sector gSec, units [gU0, min(gU0 + UG, gEnd))
+	constexpr int UG = group_units<MIXED, R>();
+	const uint32_t gi = (uint32_t)tid % a.gGroups;
+	const uint32_t gSub = (uint32_t)tid / a.gGroups;
+	const bool gOn = gSub < a.gSubs;
+	const int gSec = gi >= a.gGroups0 ? 1 : 0;
+	const uint32_t gU0 = gSec ? a.secUnit + (gi - a.gGroups0) * UG : gi * UG;
+	const uint32_t gEnd = gSec ? a.rowUnits : a.secUnit;
+
+	double2 G[UG][P][R];
+#pragma unroll
+	for (int j = 0; j < UG; ++j)
+#pragma unroll
+		for (int k = 0; k < P; ++k)
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				G[j][k][r] = make_double2(0., 0.);
+	double lnS = 0., lnC = 0., wsum = 0.;
+
+	// Every thread issues kMaxLoads loads per tile, so that the staging registers
+	// are a fixed set: a load past the tile repeats the thread's first unit, and
+	// indices past the array are clamped to its last unit (rows past the last
+	// event hold copies of real amplitudes and get weight zero).  The loads of
+	// the next tile are in flight while a tile is computed (the fetch after the
+	// block's last tile re-reads the array's last tile, so that every stage
+	// write waits for the same loads).
+	f32x4 buf[kMaxLoads];
+#define RPWA_STAGE_FETCH(T_)                                                           \
+	do {                                                                               \
+		const uint64_t base_ = (uint64_t)(T_) * a.tileUnits;                          \
+		_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k) {                        \
+			const uint64_t g_ = base_ + ldOff[k];                                      \
+			buf[k] = a.amps[g_ < a.lastUnit ? g_ : a.lastUnit];                        \
+		}                                                                              \
+	} while (0)
+
+	const uint32_t step = gridDim.x;
+	const uint32_t lastTile = a.nTiles ? a.nTiles - 1 : 0;
+	uint32_t tile = blockIdx.x;
+	if (tile < a.nTiles) {
+		RPWA_STAGE_FETCH(tile);
+		const uint32_t nMine = (a.nTiles - 1 - tile) / step + 1;  // tiles of this workgroup
+		for (uint32_t i = 0; i < nMine; ++i, tile += step) {
+			lds_barrier();  // the previous tile's G phase is done with the stage
+#pragma unroll
+			for (int k = 0; k < kMaxLoads; ++k)
+				stage[ldsPos[k]] = buf[k];
+			RPWA_STAGE_FETCH(min(tile + step, lastTile));
+			lds_barrier();
+			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gEnd, gSub, gSec, G, lnS, lnC,
+			                         wsum);
+		}
+	}
+#undef RPWA_STAGE_FETCH
+
+
+	// ---- workgroup partial row ----
+	lds_barrier();
+	double2* gbuf = reinterpret_cast<double2*>(smem);  // [gSubs][rowUnits][P][R]
+	double* scal = reinterpret_cast<double*>(gbuf + (size_t)a.gSubs * a.rowUnits * P * R);  // [E][3]
+	if (gOn) {
+#pragma unroll
+		for (int j = 0; j < UG; ++j) {
+			const uint32_t unit = gU0 + (uint32_t)j;
+			if (unit < gEnd) {
+				double2* p = gbuf + ((size_t)gSub * a.rowUnits + unit) * P * R;
+#pragma unroll
+				for (int k = 0; k < P; ++k)
+#pragma unroll
+					for (int r = 0; r < R; ++r)
+						p[k * R + r] = G[j][k][r];
+			}
+		}
+	}
+	if (wid == 0 && (E == 64 || lane < E)) {
+		scal[lane * 3 + 0] = lnS;
+		scal[lane * 3 + 1] = -lnC;  // Kahan keeps c = (rounded - exact); value = s - c
+		scal[lane * 3 + 2] = wsum;
+	}
+	lds_barrier();
+	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
+	if (tid == 0) {
+		double s = 0., c = 0., ws = 0.;
+		for (int e = 0; e < E; ++e) {
+			ek::neumaier_add(s, c, scal[e * 3 + 0]);
+			ek::neumaier_add(s, c, scal[e * 3 + 1]);
+			ws += scal[e * 3 + 2];
+		}
+		out[0] = s;
+		out[1] = c;
+		out[2] = ws;
+	}
+	// element 3 + 2*((r*2 + s)*wmax + j) (+1 for the imaginary part)
+	const uint32_t nCplx = (a.nElem - 3) / 2;
+	for (uint32_t idx = tid; idx < nCplx; idx += kThreads) {
+		const uint32_t j = idx % a.wmax;
+		const uint32_t rs = idx / a.wmax;
+		const int s = (int)(rs & 1);
+		const int r = (int)(rs >> 1);
+		double2 v = make_double2(0., 0.);
+		if (r < R && j < (s ? a.nW1 : a.nW0)) {
+			const uint32_t c = (s ? a.secOff1 : 0u) + j;
+			const uint32_t unit = c / P;
+			const uint32_t k = c % P;
+			for (uint32_t sub = 0; sub < a.gSubs; ++sub) {
+				const double2 q = gbuf[((size_t)sub * a.rowUnits + unit) * P * R + k * R + r];
+				v.x += q.x;
+				v.y += q.y;
+			}
+		}
+		out[3 + 2 * idx] = v.x;
+		out[3 + 2 * idx + 1] = v.y;
+	}
+}
+
+using StageKernel = void (*)(StageArgs);
+
+template <bool MIXED, int E>
+StageKernel pick_r(int rank)
+{
+	if (rank <= 1)
+		return stage_kernel<MIXED, 1, E>;
+	if (rank <= 2)
+		return stage_kernel<MIXED, 2, E>;
+	return stage_kernel<MIXED, 4, E>;
+}
+
+template <bool MIXED>
+StageKernel pick_e(int events, int rank)
+{
+	switch (events) {
+	case 64: return pick_r<MIXED, 64>(rank);
+	case 32: return pick_r<MIXED, 32>(rank);
+	default: return pick_r<MIXED, 16>(rank);
+	}
+}
+
+StageKernel pick_stage(const AmpLayout& lay, int events, int rank)
+{
+	return lay.precision == RPWA_PREC_MIXED ? pick_e<true>(events, rank) : pick_e<false>(events, rank);
+}
+
+}  // namespace
+
+int stage_rank_cap(int rank) { return rank <= 1 ? 1 : (rank <= 2 ? 2 : 4); }
+
+bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
+{
+	if (events != 16 && events != 32 && events != 64)
+		return false;
+	const uint32_t bpc = lay.precision == RPWA_PREC_MIXED ? 8u : 16u;
+	const uint32_t P = 16u / bpc;
+	p.events = events;
+	p.rowUnits = lay.rowLen * bpc / 16u;
+	if (p.rowUnits == 0 || p.rowUnits > (uint32_t)kThreads)
+		return false;
+	const uint32_t tileUnits = (uint32_t)events * p.rowUnits;
+	if (tileUnits > (uint32_t)(kThreads * kMaxLoads))
+		return false;
+	p.rowUnitsP = p.rowUnits | 1u;
+	p.loads = (tileUnits + kThreads - 1) / kThreads;
+	p.ranks = stage_rank_cap(rank);
+	const bool mixed = lay.precision == RPWA_PREC_MIXED;
+	const uint32_t ug = (uint32_t)(mixed ? (p.ranks == 1 ? group_units<true, 1>() : p.ranks == 2 ? group_units<true, 2>()
+	                                                                                     : group_units<true, 4>())
+	                                     : (p.ranks == 1 ? group_units<false, 1>() : p.ranks == 2 ? group_units<false, 2>()
+	                                                                                     : group_units<false, 4>()));
+	const uint32_t secUnit = lay.secOff[1] * bpc / 16u;
+	p.gGroups0 = (secUnit + ug - 1) / ug;
+	p.gGroups = p.gGroups0 + (p.rowUnits - secUnit + ug - 1) / ug;
+	if (p.gGroups > (uint32_t)kThreads)
+		return false;
+	p.gSubs = (uint32_t)kThreads / p.gGroups;
+	p.stageBytes = ((uint32_t)events * p.rowUnitsP + 1u) * 16u;
+	const int ps = p.ranks == 1 ? part_stride<1>() : p.ranks == 2 ? part_stride<2>() : part_stride<4>();
+	const size_t partBytes = (size_t)(kWaves - 1) * events * ps * sizeof(double2);
+	const size_t endBytes = (size_t)p.gSubs * p.rowUnits * P * p.ranks * sizeof(double2) +
+	                        (size_t)events * 3 * sizeof(double);
+	p.ldsBytes = std::max((size_t)p.stageBytes + partBytes, endBytes);
+	return p.ldsBytes <= 160u * 1024u;
+}
+
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& ea, int nWg, hipStream_t s)
+{
+	StagePlan p;
+	if (!stage_plan(lay, events, ea.rank, p))
+		return hipErrorInvalidValue;
+	StageArgs a;
+	a.amps = reinterpret_cast<const f32x4*>(ea.amps);
+	a.nEvents = lay.nEvents;
+	a.lastUnit = lay.nEvents ? lay.nEvents * p.rowUnits - 1 : 0;  // no tile, no load when there are no events
+	a.nTiles = (uint32_t)((lay.nEvents + events - 1) / events);
+	a.rowUnits = p.rowUnits;
+	a.rowUnitsP = p.rowUnitsP;
+	a.tileUnits = (uint32_t)events * p.rowUnits;
+	a.loads = p.loads;
+	a.sliceUnits = (p.rowUnits + kWaves - 1) / kWaves;
+	const uint32_t bpc = lay.precision == RPWA_PREC_MIXED ? 8u : 16u;
+	a.secUnit = lay.secOff[1] * bpc / 16u;
+	a.gGroups0 = p.gGroups0;
+	a.gGroups = p.gGroups;
+	a.gSubs = p.gSubs;
+	a.stageBytes = p.stageBytes;
+	a.secOff1 = lay.secOff[1];
+	a.nW0 = lay.nW[0];
+	a.nW1 = lay.nW[1];
+	a.wmax = lay.wmax;
+	a.tpad = ea.tpad;
+	a.flat2 = ea.flat2;
+	a.partials = ea.partials;
+	a.nElem = ea.nElem;
+	StageKernel k = pick_stage(lay, events, ea.rank);
+	if (p.ldsBytes > 64u * 1024u) {
+		hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.ldsBytes);
+		if (e != hipSuccess)
+			return e;
+	}
+	hipLaunchKernelGGL(k, dim3(nWg), dim3(kThreads), p.ldsBytes, s, a);
+	return hipGetLastError();
+}
+
+int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu)
+{
+	StagePlan p;
+	if (!stage_plan(lay, events, rank, p))
+		return RPWA_ERR_UNSUPPORTED;
+	StageKernel k = pick_stage(lay, events, rank);
+	if (p.ldsBytes > 64u * 1024u)
+		if (hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+		                        (int)p.ldsBytes) != hipSuccess)
+			return RPWA_ERR_HIP;
+	int n = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, p.ldsBytes) != hipSuccess)
+		return RPWA_ERR_HIP;
+	*wgPerCu = n > 0 ? n : 1;
+	return RPWA_OK;
+}
+
+}  // namespace rpwa_amd

@@ -61,23 +61,34 @@ def test_sign_flip_and_determinism(gpu, case, prec):
             assert ctx.log_likelihood(prod, -0.2) == f0
 
 
-def test_every_tile_shape_agrees(gpu, case):
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_every_tile_shape_agrees(gpu, case, prec):
+    """Every kernel variant (lanes per event 8..64 of the register-tile kernels,
+    1 = the LDS-staged kernel) against the oracle, with the bitwise contracts
+    (value-only == fused, sign flip of a (rank, sector) block) per variant."""
     ds, L, amps = case
     nW, rank = ds["nW"], ds["rank"]
-    ctx = gpu.LikelihoodContext(0, 0)
+    ctx = gpu.LikelihoodContext(0, prec)
     ctx.load(amps, nW)
     prod = H.random_prod(np.random.default_rng(3), rank, nW, 0.9)
     f_ref = L.event_loglik(prod, 0.1)
-    d_ref, _ = L.event_deriv(prod, 0.1)
-    for lanes in (8, 16, 32, 64):
+    d_ref, df_ref = L.event_deriv(prod, 0.1)
+    tol_f, tol_d = (1e-12, 1e-10) if prec == 0 else (1e-6, 1e-5)
+    for lanes in (1, 8, 16, 32, 64):
         try:
             ctx.set_tile(lanes)
         except gpu.RpwaError as e:
             assert e.code == gpu.RPWA_ERR_UNSUPPORTED
             continue
-        f, d, _ = ctx.fdf(prod, 0.1)
-        assert abs(f - f_ref) <= 1e-12 * abs(f_ref), lanes
-        assert H.rel_err(d, d_ref) <= 1e-10, lanes
+        print("tile", lanes, "lanes per event", flush=True)
+        f, d, df = ctx.fdf(prod, 0.1)
+        assert abs(f - f_ref) <= tol_f * abs(f_ref), lanes
+        assert H.rel_err(d, d_ref) <= tol_d, lanes
+        assert abs(df - df_ref) <= tol_d * abs(df_ref), lanes
+        assert ctx.log_likelihood(prod, 0.1) == f
+        p2 = prod.copy()
+        p2[rank - 1, 1] *= -1
+        assert ctx.log_likelihood(p2, 0.1) == f
 
 
 def test_reshuffled_layout(gpu):
