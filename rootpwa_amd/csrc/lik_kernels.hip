@@ -44,9 +44,10 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double x)
 }
 
 // Second stage: fixed-order sum of the workgroup partial rows.
-// Block = 64 elements x 16 chunks of workgroups; chunks are combined in order.
-constexpr int kRedElems = 64;
-constexpr int kRedChunks = 16;
+// Block = 16 elements x 64 chunks of workgroups (a few rows per thread, so the
+// sum is not bound by one thread's load latency); chunks are combined in order.
+constexpr int kRedElems = 16;
+constexpr int kRedChunks = 64;
 
 __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
     const double* __restrict__ partials, int nWg, uint32_t nElem, double flat,
