@@ -53,6 +53,15 @@ def measured_traffic(N, W):
     return {}
 
 
+def kernel_label(info, precision, rank):
+    """Name of the event-kernel variant the context's autotuner kept."""
+    if info["lanes"] == 1:
+        return "stage_kernel<%s, rank %d, %d events per tile> (LDS-staged, fused lnL + gradient)" % (
+            precision, rank, info["slots"])
+    return "event_kernel<%s, %d lanes, %d slots, rank %d> (register tiles, fused lnL + gradient)" % (
+        precision, info["lanes"], info["slots"], rank)
+
+
 def start_values(rng, nPars, nEvents, W):
     """Random start at the scale of the fit (|T|^2 ~ N/W per wave)."""
     return rng.normal(size=nPars) * np.sqrt(nEvents / W)
@@ -255,10 +264,11 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": ach_mix, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_mix / HBM_PEAK_GBS, "traffic": traffic.get("mixed"),
-                     "kernel": "event_kernel<mixed> (fused lnL + gradient)",
+                     "kernel": kernel_label(r_mix["info"], "mixed", rank),
                      "kernel_ms": 1e3 * r_mix["kernel_s"], "bytes_per_launch": bytes_mix,
                      "measured_read_GBs": read_gbs, "frac_of_measured_read": ach_mix / read_gbs},
         "fp64": {"evals_per_s": r_64["rate"], "ms_per_eval": r_64["ms"], "kernel_ms": 1e3 * r_64["kernel_s"],
+                 "kernel": kernel_label(r_64["info"], "fp64", rank),
                  "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs,
                  "traffic": traffic.get("fp64")},
         "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
