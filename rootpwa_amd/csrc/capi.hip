@@ -128,7 +128,6 @@ void free_buffers(rpwa_lik_ctx* c)
 	hipFree(c->d_amps);
 	hipFree(c->d_prod);
 	hipFree(c->d_partials);
-	hipFree(c->d_out);
 	hipHostFree(c->h_prod);
 	hipHostFree(c->h_out);
 	c->d_amps = nullptr;
@@ -196,9 +195,10 @@ int alloc_eval_buffers(rpwa_lik_ctx* c)
 	const size_t nProd = (size_t)kMaxRank * 2 * c->lay.wmax + (size_t)c->lay.rowLen * kMaxRank;
 	const size_t nOut = partial_elems(kMaxRank, c->lay.wmax);
 	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
-	RPWA_HIP_CHECK(hipMalloc(&c->d_out, nOut * sizeof(double)));
 	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
-	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocDefault));
+	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command)
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+	RPWA_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_out), c->h_out, 0));
 	for (int r = 1; r <= kMaxRank; ++r)
 		if (int st = configure_grid(c, r))
 			return st;
@@ -260,6 +260,7 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	a.rank = rank;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
 	a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nProd;
+	a.tpadHost = prod.data() + nProd;
 	a.flat2 = 0.01;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
@@ -337,14 +338,17 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	}
 	std::memcpy(c->h_prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2));
 	size_t nUp = nmbProdAmps;
-	if (c->tile[rank].events > 0) {
+	const bool staged = c->tile[rank].events > 0;
+	if (staged) {
 		const int ranks = stage_rank_cap((int)rank);
 		fill_tpad(c->lay, reinterpret_cast<const double2*>(c->h_prod), (int)rank, ranks,
 		          reinterpret_cast<double2*>(c->h_prod) + nmbProdAmps);
 		nUp += (size_t)c->lay.rowLen * ranks;
 	}
-	RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, nUp * sizeof(double2), hipMemcpyHostToDevice,
-	                              c->stream));
+	// the staged kernel takes T in its arguments when it fits: nothing to upload
+	if (!(staged && stage_t_inline(c->lay, (int)rank)))
+		RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, nUp * sizeof(double2), hipMemcpyHostToDevice,
+		                              c->stream));
 	const uint32_t nElem = partial_elems((int)rank, wmax);
 	EventArgs a;
 	a.amps = c->d_amps;
@@ -360,6 +364,7 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.lanes1 = c->tile[rank].lanes1;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
 	a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps;
+	a.tpadHost = reinterpret_cast<const double2*>(c->h_prod) + nmbProdAmps;
 	a.flat2 = flat * flat;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
@@ -368,8 +373,6 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, flat, c->d_out, c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-	RPWA_HIP_CHECK(hipMemcpyAsync(c->h_out, c->d_out, (nElem - 1) * sizeof(double), hipMemcpyDeviceToHost,
-	                              c->stream));
 	c->pending = true;
 	c->pendingRank = rank;
 	c->pendingProd = nmbProdAmps;

@@ -79,6 +79,7 @@ struct EventArgs {
 	int lanes0, lanes1;
 	const double2* prod;   // device [rank][2][wmax] complex
 	const double2* tpad;   // device [rowLen][ranks of the staged kernel]: T per row column, 0 on padding
+	const double2* tpadHost;  // the same on the host (the staged kernel takes it in its arguments when it fits)
 	double flat2;
 	double* partials;      // [nWg][nElem]
 	uint32_t nElem;
@@ -93,6 +94,8 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
                                bool grad, int nWg, hipStream_t s);
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
 bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p);
+// true if T of every row column fits into the staged kernel's arguments (no upload needed)
+bool stage_t_inline(const AmpLayout& lay, int rank);
 int stage_rank_cap(int rank);
 hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s);
 int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu);
@@ -124,9 +127,9 @@ struct rpwa_lik_ctx {
 	void* d_amps = nullptr;
 	double* d_prod = nullptr;      // [kMaxRank][2][wmax] complex
 	double* d_partials = nullptr;  // [nWg][partial_elems]
-	double* d_out = nullptr;       // reduced results
+	double* d_out = nullptr;       // device view of h_out: the reduce kernel writes the results there
 	double* h_prod = nullptr;      // pinned staging
-	double* h_out = nullptr;       // pinned staging
+	double* h_out = nullptr;       // reduced results (pinned, coherent, mapped into the device)
 	size_t partialCap = 0;
 	// pending asynchronous evaluation (rpwa_lik_submit / rpwa_lik_collect)
 	bool pending = false;

@@ -28,12 +28,15 @@
 // reproducible and invariant under T -> -T of a (rank, sector) block.
 #include "event_kernel.h"
 
+#include <cstddef>
+
 namespace rpwa_amd {
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxLoads = 12;  // 16-byte loads per thread and tile (E * rowUnits <= 3072)
+constexpr int kInlineT = 192;  // complex values of T that travel in the kernel arguments (3 KB)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));  // one 16-byte unit
 
@@ -73,6 +76,8 @@ struct StageArgs {
 	double flat2;
 	double* partials;       // [nWg][nElem]
 	uint32_t nElem;
+	uint32_t tInline;       // 1: T is tin below (read from the kernel arguments), not tpad
+	double2 tin[kInlineT];
 };
 
 // all LDS writes of this wave done, then the workgroup barrier (no wait for
@@ -95,8 +100,8 @@ __device__ __forceinline__ double2 unit_cplx(const f32x4 v, int k)
 // threads).  The stage holds the tile; part is the LDS exchange area.
 template <bool MIXED, int R, int E>
 __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* stage, double2* part, int lane,
-                                            int wid, uint32_t tile, bool gOn, uint32_t gU0, uint32_t gEnd,
-                                            uint32_t gSub, int gSec,
+                                            int wid, uint32_t tile, bool gOn, uint32_t gU0, uint32_t gSub,
+                                            int gSec,
                                             double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R],
                                             double& lnS, double& lnC, double& wsum)
 {
@@ -115,7 +120,10 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 		const uint32_t u0 = (uint32_t)wid * a.sliceUnits;
 		const uint32_t u1 = min(u0 + a.sliceUnits, a.rowUnits);
 		const uint32_t um = min(max(a.secUnit, u0), u1);
-		cdouble* tp = (cdouble*)(const double*)a.tpad;
+		// T from the kernel arguments (the argument block starts the kernarg segment) or from memory
+		typedef const __attribute__((address_space(4))) char cchar;
+		cdouble* tp = a.tInline ? (cdouble*)((cchar*)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(StageArgs, tin))
+		                        : (cdouble*)(const double*)a.tpad;
 #pragma unroll 4
 		for (uint32_t u = u0; u < um; ++u) {
 			const f32x4 v = row[u];
@@ -211,7 +219,6 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 			}
 		}
 	}
-	(void)gEnd;
 }
 
 template <bool MIXED, int R, int E>
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 				stage[ldsPos[k]] = buf[k];
 			RPWA_STAGE_FETCH(min(tile + step, lastTile));
 			lds_barrier();
-			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gEnd, gSub, gSec, G, lnS, lnC,
+			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gSub, gSec, G, lnS, lnC,
 			                         wsum);
 		}
 	}
@@ -387,6 +394,11 @@ StageKernel pick_stage(const AmpLayout& lay, int events, int rank)
 
 int stage_rank_cap(int rank) { return rank <= 1 ? 1 : (rank <= 2 ? 2 : 4); }
 
+bool stage_t_inline(const AmpLayout& lay, int rank)
+{
+	return (size_t)lay.rowLen * stage_rank_cap(rank) <= (size_t)kInlineT;
+}
+
 bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 {
 	if (events != 16 && events != 32 && events != 64)
@@ -452,6 +464,13 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 	a.flat2 = ea.flat2;
 	a.partials = ea.partials;
 	a.nElem = ea.nElem;
+	a.tInline = 0;
+	if (ea.tpadHost && stage_t_inline(lay, ea.rank)) {
+		a.tInline = 1;
+		const size_t n = (size_t)lay.rowLen * stage_rank_cap(ea.rank);
+		for (size_t i = 0; i < n; ++i)
+			a.tin[i] = ea.tpadHost[i];
+	}
 	StageKernel k = pick_stage(lay, events, ea.rank);
 	if (p.ldsBytes > 64u * 1024u) {
 		hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),

@@ -8,7 +8,9 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [((0, 16), 1), ((3, 5), 2), ((8, 80), 2), ((7, 34), 2), ((1, 1), 1), ((5, 0), 1), ((4, 9), 3)]
+# (refl -1 waves, refl +1 waves), rank; (30, 170) makes the LDS-staged kernel use 16-event tiles
+SHAPES = [((0, 16), 1), ((3, 5), 2), ((8, 80), 2), ((7, 34), 2), ((1, 1), 1), ((5, 0), 1), ((4, 9), 3),
+          ((30, 170), 2)]
 
 
 @pytest.fixture(scope="module", params=SHAPES, ids=lambda p: "W%d+%d_r%d" % (p[0][0], p[0][1], p[1]))
