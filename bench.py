@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--skip-oracle", action="store_true", help="skip the full-size oracle check (profiling runs)")
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched-fit leg")
+    ap.add_argument("--skip-hessian", action="store_true", help="skip the analytic-Hessian leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="profiling runs: only the batched leg, printed as its own JSON line")
     ap.add_argument("--initializer", action="store_true",
@@ -190,6 +191,7 @@ def main():
 
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
+    hess = None if args.skip_hessian else hessian_leg(L_64, L_mix, par, N, W, rank, log)
 
     # ---- checks against the oracle at full size ---------------------------------
     check = {}
@@ -278,10 +280,43 @@ def main():
         "check": check,
         "cpu_baseline": cpu,
         "batched": batched,
+        "hessian": hess,
     }
     if cpu:
         out["vs_cpu"] = r_mix["rate"] / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def hessian_leg(L_64, L_mix, par, N, W, rank, log):
+    """Analytic Hessian of -lnL (pwaLikelihood::Hessian) at the benchmark point on
+    the full workload, FP64 and mixed: wall time of one call (after one untimed
+    call), and a size-independent check: H v against the central difference of
+    the analytic gradient along a random direction v (FdF at par +- eps v)."""
+    from rootpwa_amd import capi
+    out = {}
+    rng = np.random.default_rng(SEED + 7)
+    v = rng.normal(size=par.size)
+    v /= np.linalg.norm(v)
+    for label, L in (("fp64", L_64), ("mixed", L_mix)):
+        L.Hessian(par)
+        capi.synchronize()
+        t = time.perf_counter()
+        H = L.Hessian(par)
+        capi.synchronize()
+        el = time.perf_counter() - t
+        eps = 1e-3 * np.sqrt(N / W)
+        gp = L.FdF(par + eps * v)[1]
+        gm = L.FdF(par - eps * v)[1]
+        fd = (gp - gm) / (2 * eps)
+        hv = H @ v
+        out[label] = {"ms": 1e3 * el, "asymmetry": float(np.max(np.abs(H - H.T)) / np.max(np.abs(H))),
+                      "hv_vs_gradient_difference": float(np.max(np.abs(hv - fd)) / np.max(np.abs(fd)))}
+        log("hessian %s: %.1f ms per call (%d x %d), |Hv - dg|/|dg| = %.2e" % (
+            label, 1e3 * el, par.size, par.size, out[label]["hv_vs_gradient_difference"]))
+    out["parameters"] = int(par.size)
+    out["events"] = int(N)
+    out["rank"] = rank
+    return out
 
 
 def initializer_leg(args, log):
