@@ -340,13 +340,28 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 	const uint32_t c0 = r.binChunk0[b], c1 = r.binChunk0[b + 1];
 	const size_t fit = blockIdx.x;
 	const uint32_t i = threadIdx.x;
-	__shared__ double red[128];
+	__shared__ double red[3][128];
 	const float* Tf = r.T32 + fit * r.rowLen;
 	double normPart = 0.;
 	if (i < r.W) {
+		// chunk partials of wave component i, added in chunk order; the loads of
+		// eight chunks are issued before their adds
 		double gre = 0., gim = 0.;
-		for (uint32_t c = c0; c < c1; ++c) {
-			const float* pg = r.partG + ((size_t)(c * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
+		const size_t cstride = (size_t)4 * r.Wp * 2 * kGroup;  // floats between consecutive chunks
+		const float* pg = r.partG + ((size_t)(c0 * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
+		uint32_t c = c0;
+		for (; c + 8 <= c1; c += 8, pg += 8 * cstride) {
+			float2 v[8];
+#pragma unroll
+			for (int k = 0; k < 8; ++k)
+				v[k] = *reinterpret_cast<const float2*>(pg + k * cstride);
+#pragma unroll
+			for (int k = 0; k < 8; ++k) {
+				gre += (double)v[k].x;
+				gim += (double)v[k].y;
+			}
+		}
+		for (; c < c1; ++c, pg += cstride) {
 			gre += (double)pg[0];
 			gim += (double)pg[1];
 		}
@@ -370,22 +385,29 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 			gout[2 * i] = dim;
 		}
 	}
-	red[i] = normPart;
+	// the per-chunk log sums and weight sums: thread i takes chunks c0+i, c0+i+128, ...
+	double lnPart = 0., invPart = 0.;
+	for (uint32_t c = c0 + i; c < c1; c += 128) {
+		lnPart += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
+		invPart += r.partInv[(size_t)(c * 4 + q) * kGroup + jc];
+	}
+	red[0][i] = normPart;
+	red[1][i] = lnPart;
+	red[2][i] = invPart;
 	__syncthreads();
-	for (uint32_t w = 64; w > 0; w >>= 1) {
-		if (i < w)
-			red[i] += red[i + w];
+	for (uint32_t w = 64; w > 0; w >>= 1) {  // fixed-order tree over the 128 threads
+		if (i < w) {
+			red[0][i] += red[0][i + w];
+			red[1][i] += red[1][i + w];
+			red[2][i] += red[2][i + w];
+		}
 		__syncthreads();
 	}
 	if (i == 0) {
-		double lnSum = 0., invSum = 0.;
-		for (uint32_t c = c0; c < c1; ++c) {
-			lnSum += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
-			invSum += r.partInv[(size_t)(c * 4 + q) * kGroup + jc];
-		}
+		const double lnSum = red[1][0], invSum = red[2][0];
 		const double flat = r.xt[fit * r.P + r.P - 1];
 		const double eta = r.totAcc[b];
-		r.ft[fit] = -lnSum + (red[0] + flat * flat * eta);
+		r.ft[fit] = -lnSum + (red[0][0] + flat * flat * eta);
 		r.gt[fit * r.P + r.P - 1] = -2. * flat * invSum + 2. * flat * eta;
 	}
 }
