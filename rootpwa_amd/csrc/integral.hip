@@ -9,7 +9,7 @@
 //   Re I_ij = C[2i][2j] + C[2i+1][2j+1],   Im I_ij = C[2i+1][2j] - C[2i][2j+1].
 // C is computed as a rank-k update on v_mfma_f64_16x16x4_f64: 16x16 output
 // tiles of the upper triangle (tile row <= tile column), k = events.
-// A persistent grid of one 256-thread workgroup per CU walks 16-event chunks
+// A persistent grid of one 512-thread workgroup (8 waves) per CU walks event chunks
 // (grid-stride); each chunk is staged once through LDS and every wave issues
 // the MFMAs of its tiles from it.  Each workgroup writes its tile sums to a
 // partial row; partial rows are summed in fixed order (integral_reduce), and
@@ -26,10 +26,10 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kIntThreads = 256;
-constexpr int kIntWaves = 4;
-constexpr int kMaxTilesPerWave = 24;
-constexpr int kMaxLoadsPerThread = 16;  // double2 loads per thread and chunk: CH * D / 2 <= 4096
+constexpr int kIntThreads = 512;
+constexpr int kIntWaves = 8;
+constexpr int kMaxTilesPerWave = 12;  // accumulators of 12 tiles per wave stay in registers at 2 waves per SIMD
+constexpr int kMaxLoadsPerThread = 8;  // double2 loads per thread and chunk: CH * D / 2 <= 4096
 
 struct IntArgs {
 	const double* x;       // [M][D] row-major (interleaved complex rows)
@@ -268,9 +268,7 @@ static IntKernel pick_int_nt(uint32_t per)
 	if (per <= 4) return integral_kernel<4, CH>;
 	if (per <= 6) return integral_kernel<6, CH>;
 	if (per <= 9) return integral_kernel<9, CH>;
-	if (per <= 12) return integral_kernel<12, CH>;
-	if (per <= 17) return integral_kernel<17, CH>;
-	return integral_kernel<24, CH>;
+	return integral_kernel<12, CH>;
 }
 
 static IntKernel pick_int_kernel(uint32_t nTiles, int chunk)
