@@ -60,7 +60,6 @@ struct StagePlan {
 	int ranks = 1;           // ranks held by the kernel instantiation (1, 2 or 4)
 	uint32_t rowUnits = 0;   // 16-byte units per amplitude row
 	uint32_t rowUnitsP = 0;  // LDS row stride (odd)
-	uint32_t loads = 0;      // 16-byte loads per thread and tile
 	uint32_t gGroups0 = 0;   // unit groups of the refl -1 sector in the gradient phase
 	uint32_t gGroups = 0;    // unit groups of both sectors
 	uint32_t gSubs = 0;      // event subsets per unit group in the gradient phase

@@ -14,13 +14,15 @@
 //     banks;
 //   * S phase: wave w contracts column slice w of every row, lane = event.
 //     The production amplitudes of a column are the same for every lane and
-//     are read through the scalar unit; waves 1..3 leave their partial S in
-//     LDS and wave 0 adds them in the order 0,1,2,3;
+//     are read through the scalar unit, from the kernel arguments when they
+//     fit (3 KB), else from memory; waves 1..3 leave their partial S in LDS
+//     and wave 0 adds them in the order 0,1,2,3;
 //   * sigma phase (wave 0, lane = event): sigma, -ln sigma into a per-lane
 //     Kahan sum, w = -2/sigma, and u = w S stored in LDS;
-//   * G phase: thread = (16-byte column unit, event subset); it adds
-//     u conj(A) of its unit over its events of the tile in FP64 and keeps the
-//     sums across tiles;
+//   * G phase: thread = (group of 2-4 consecutive 16-byte units of one sector,
+//     event subset); it reads u of an event once for its group, adds
+//     u conj(A) over its events of the tile in FP64 and keeps the sums across
+//     tiles;
 //   * end: the event subsets are added in fixed order and written as the
 //     workgroup's partial row, in the format of the register-tile kernels, so
 //     event_reduce and everything after it is shared.
@@ -64,7 +66,6 @@ struct StageArgs {
 	uint32_t rowUnits;      // 16-byte units per row
 	uint32_t rowUnitsP;     // LDS row stride in units (odd)
 	uint32_t tileUnits;     // E * rowUnits
-	uint32_t loads;         // ceil(tileUnits / 256)
 	uint32_t sliceUnits;    // units per wave in the S phase
 	uint32_t secUnit;       // first unit of the refl +1 sector
 	uint32_t gGroups0;      // G-phase unit groups of the refl -1 sector
@@ -413,7 +414,6 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 	if (tileUnits > (uint32_t)(kThreads * kMaxLoads))
 		return false;
 	p.rowUnitsP = p.rowUnits | 1u;
-	p.loads = (tileUnits + kThreads - 1) / kThreads;
 	p.ranks = stage_rank_cap(rank);
 	const bool mixed = lay.precision == RPWA_PREC_MIXED;
 	const uint32_t ug = (uint32_t)(mixed ? (p.ranks == 1 ? group_units<true, 1>() : p.ranks == 2 ? group_units<true, 2>()
@@ -448,7 +448,6 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 	a.rowUnits = p.rowUnits;
 	a.rowUnitsP = p.rowUnitsP;
 	a.tileUnits = (uint32_t)events * p.rowUnits;
-	a.loads = p.loads;
 	a.sliceUnits = (p.rowUnits + kWaves - 1) / kWaves;
 	const uint32_t bpc = lay.precision == RPWA_PREC_MIXED ? 8u : 16u;
 	a.secUnit = lay.secOff[1] * bpc / 16u;
