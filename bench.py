@@ -53,6 +53,77 @@ def measured_traffic(N, W):
     return {}
 
 
+def cpu_model():
+    """Host CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return "%s, %d usable cores" % (model, len(os.sched_getaffinity(0)))
+
+
+def c0_leg(args, log):
+    """The CPU-runnable configuration: 16 waves in one sector, rank 1, 200k MC
+    events, 100k accept-rejected data events; one FP64 FdF at a random point on
+    the GPU (rate over repeated calls) and in the single-thread oracle (median of
+    5 after a warm-up, full event count), with the value and gradient compared."""
+    from oracle import oracle as O
+    from rootpwa_amd import capi
+    from rootpwa_amd import likelihood as PL
+    nW, rank, N, M = (0, 16), 1, 100_000, 200_000
+    W = sum(nW)
+    seed = 20261202 + 0
+    scales = capi.gen_wave_scales(seed, W)
+    truth = capi.gen_prod_amps(seed, STREAM_TRUE_T, rank, nW)
+    d_mc = capi.DeviceBuffer(M * W * 16)
+    capi.gen_amps_device(d_mc.address, seed, STREAM_MC, 0, M, scales)
+    sums = capi.integral_device(d_mc.address, M, W)
+    d_mc.free()
+    d_data = capi.DeviceBuffer(N * W * 16)
+    capi.gen_accept_reject_device(d_data.address, seed, STREAM_DATA, N, nW, rank, truth, scales)
+    waves = wave_list(nW)
+    intmat = PL.ampIntegralMatrix([w[0] for w in waves])
+    intmat.setSums(sums, M)
+    L = PL.pwaLikelihood(capi.PREC_FP64)
+    L.init(waves, rank)
+    L.addNormIntegral(intmat)
+    L.addAccIntegral(intmat)
+    L.addAmplitudesDevice(d_data.address, N)
+    L.finishInit()
+    par = np.random.default_rng(seed + STREAM_START).normal(size=L.nmbPars()) * np.sqrt(N / W)
+    for _ in range(10):
+        f, g = L.FdF(par)
+    n = 500
+    t = time.perf_counter()
+    for _ in range(n):
+        f, g = L.FdF(par)
+    gpu_rate = n / (time.perf_counter() - t)
+    data = d_data.download((N, W), np.complex128)
+    d_data.free()
+    Lo = O.OracleLikelihood(nW, rank)
+    Lo.add_integrals(sums, M)
+    Lo.add_amplitudes(data)
+    Lo.finish_init()
+    f_ref, g_ref = Lo.fdf(par)
+    ts = []
+    for _ in range(5):
+        t = time.perf_counter()
+        Lo.fdf(par)
+        ts.append(time.perf_counter() - t)
+    cpu_s = float(np.median(ts))
+    out = {"workload": "C0: 100k accept-rejected events x 16 waves (one sector), rank 1, 200k-event MC; FP64 FdF",
+           "parameters": int(L.nmbPars()), "gpu_evals_per_s": gpu_rate, "cpu_evals_per_s": 1.0 / cpu_s,
+           "cpu_cores": 1, "cpu_kind": "port (single-thread C oracle, median of 5 after a warm-up)",
+           "rel_lnL": abs(f - f_ref) / abs(f_ref),
+           "grad_rel": float(np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref)))}
+    log("C0:", out)
+    return out
+
+
 def kernel_label(info, precision, rank):
     """Name of the event-kernel variant the context's autotuner kept."""
     if info["lanes"] == 1:
@@ -75,12 +146,13 @@ def main():
     ap.add_argument("--events", type=int, default=2_000_000)
     ap.add_argument("--mc-events", type=int, default=5_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="events of the single-thread CPU baseline")
-    ap.add_argument("--cpu-evals", type=int, default=3)
+    ap.add_argument("--cpu-evals", type=int, default=5)
     ap.add_argument("--oracle-threads", type=int, default=16)
     ap.add_argument("--skip-oracle", action="store_true", help="skip the full-size oracle check (profiling runs)")
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched-fit leg")
     ap.add_argument("--skip-hessian", action="store_true", help="skip the analytic-Hessian leg")
+    ap.add_argument("--skip-c0", action="store_true", help="skip the small CPU-runnable configuration")
     ap.add_argument("--only-batch", action="store_true",
                     help="profiling runs: only the batched leg, printed as its own JSON line")
     ap.add_argument("--initializer", action="store_true",
@@ -186,8 +258,17 @@ def main():
         log("%s: %.1f evals/s (%.3f ms per FdF; event kernel %.3f ms, reduce %.3f ms; tile %d lanes x %d slots, "
             "%d workgroups)" % (label, rate, 1e3 * el / args.steps, 1e3 * k_avg, 1e3 * red / max(n, 1),
                                 info["lanes"], info["slots"], info["workgroups"]))
+        # the Minuit2 pattern: DoEval then Gradient at the same point
+        n_pair = max(1, args.steps // 2)
+        capi.synchronize()
+        t = time.perf_counter()
+        for _ in range(n_pair):
+            L.DoEval(par)
+            L.Gradient(par)
+        el_pair = time.perf_counter() - t
+        log("%s: %.1f DoEval+Gradient pairs/s" % (label, n_pair / el_pair))
         return dict(rate=rate, ms=1e3 * el / args.steps, kernel_s=k_avg, reduce_s=red / max(n, 1), f=f, g=g,
-                    info=info)
+                    info=info, pair_rate=n_pair / el_pair)
 
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
@@ -216,7 +297,7 @@ def main():
         del Lo
 
     # ---- CPU baseline: the oracle, single thread, on a bounded sample ------------
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         from oracle import oracle as O
         S = min(args.cpu_sample, N)
@@ -226,17 +307,33 @@ def main():
         Lc.add_amplitudes(sub)
         Lc.finish_init()
         del sub
-        Lc.fdf(par)
-        t = time.perf_counter()
-        for _ in range(args.cpu_evals):
-            Lc.fdf(par)
-        el = (time.perf_counter() - t) / args.cpu_evals
-        cpu = dict(value=(1.0 / el) * S / N, unit="evals/s", cores=1, kind="port",
-                   sample="%d FdF calls of the single-thread C oracle (restatement of pwaLikelihood::FdF, "
-                          "Kahan sums, per-event allocation) on the first %d of the %d events, %.2f s each; "
-                          "evals/s scaled by %d/%d to the full event count" % (args.cpu_evals, S, N, el, S, N))
-        log("cpu baseline:", cpu)
+        Lc.fdf(par)  # warm-up
 
+        def median_time(fn, n):
+            ts = []
+            for _ in range(n):
+                t = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t)
+            return float(np.median(ts))
+        el = median_time(lambda: Lc.fdf(par), args.cpu_evals)
+        el_pair = median_time(lambda: (Lc.doeval(par), Lc.gradient(par)), args.cpu_evals)
+        threads = args.oracle_threads
+        el_mt = median_time(lambda: Lc.fdf(par, threads=threads), args.cpu_evals)
+        cpu = dict(value=(1.0 / el) * S / N, unit="evals/s", cores=1, kind="port",
+                   sample="median of %d FdF calls (after one warm-up) of the single-thread C oracle (restatement "
+                          "of pwaLikelihood::FdF, Kahan sums, per-event allocation) on the first %d of the %d "
+                          "events, %.3f s each; evals/s scaled by %d/%d to the full event count"
+                          % (args.cpu_evals, S, N, el, S, N),
+                   minuit_pattern_evals_per_s=(1.0 / el_pair) * S / N,
+                   cpu_model=cpu_model())
+        cpu_all = dict(value=(1.0 / el_mt) * S / N, unit="evals/s", cores=threads, kind="port",
+                       sample="the same oracle FdF with the events split into %d contiguous ranges on %d threads, "
+                              "partials merged in fixed order (median of %d, %.4f s each on %d events)"
+                              % (threads, threads, args.cpu_evals, el_mt, S))
+        log("cpu baseline:", cpu, "all cores:", cpu_all)
+
+    c0 = None if args.skip_c0 else c0_leg(args, log)
     batched = None if args.skip_batch else batched_leg(args, log)
 
     bytes_mix = float(N) * W * 8
@@ -279,6 +376,10 @@ def main():
         "bandwidth_probe": {"read_GBs": read_gbs, "copy_GBs": copy_gbs, "bytes": 4 << 30},
         "check": check,
         "cpu_baseline": cpu,
+        "cpu_all_cores": cpu_all,
+        "minuit_pattern": {"mixed_evals_per_s": r_mix["pair_rate"], "fp64_evals_per_s": r_64["pair_rate"],
+                           "note": "one DoEval + Gradient pair per eval at the same point (Minuit2 pattern)"},
+        "c0": c0,
         "batched": batched,
         "hessian": hess,
     }
