@@ -17,8 +17,10 @@
 //     are read through the scalar unit, from the kernel arguments when they
 //     fit (3 KB), else from memory; waves 1..3 leave their partial S in LDS
 //     and wave 0 adds them in the order 0,1,2,3;
-//   * sigma phase (wave 0, lane = event): sigma, -ln sigma into a per-lane
-//     Kahan sum, w = -2/sigma, and u = w S stored in LDS;
+//   * sigma phase (wave 0, lane = event): sigma, the per-lane running product
+//     of sigma as m 2^e (its log is taken once at the end: the phase is on the
+//     critical path of every tile, a log per event is not), w = -2/sigma, and
+//     u = w S stored in LDS;
 //   * G phase: thread = (group of 2-4 consecutive 16-byte units of one sector,
 //     event subset); it reads u of an event once for its group, adds
 //     u conj(A) over its events of the tile in FP64 and keeps the sums across
@@ -104,7 +106,7 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
                                             int wid, uint32_t tile, bool gOn, uint32_t gU0, uint32_t gSub,
                                             int gSec,
                                             double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R],
-                                            double& lnS, double& lnC, double& wsum)
+                                            double& mant, int& expo, double& wsum)
 {
 	constexpr int P = MIXED ? 2 : 1;
 	constexpr int UG = group_units<MIXED, R>();
@@ -181,8 +183,19 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 		sigma += a.flat2;
 		double w = 0.;
 		if ((uint64_t)tile * E + lane < a.nEvents) {
-			ek::kahan_add(lnS, lnC, -log(sigma));
-			w = -2. / sigma;
+			// sum of ln sigma as the log of a running product m 2^e, m in [1, 2)
+			// (one log per lane at the end instead of one per event)
+			mant *= sigma;
+			const uint64_t ub = (uint64_t)__double_as_longlong(mant);
+			const int ex = (int)((ub >> 52) & 0x7FF);
+			if (ex != 0 && ex != 0x7FF) {
+				expo += ex - 1023;
+				mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+			}
+			// -2/sigma from the hardware reciprocal and one Newton step
+			double r = __builtin_amdgcn_rcp(sigma);
+			r = fma(fma(-sigma, r, 1.), r, r);
+			w = -2. * r;
 			wsum += w;
 		}
 		double2* p = part + (size_t)lane * PS;  // u replaces wave 1's partial
@@ -266,7 +279,8 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 #pragma unroll
 			for (int r = 0; r < R; ++r)
 				G[j][k][r] = make_double2(0., 0.);
-	double lnS = 0., lnC = 0., wsum = 0.;
+	double mant = 1., wsum = 0.;
+	int expo = 0;
 
 	// Every thread issues kMaxLoads loads per tile, so that the staging registers
 	// are a fixed set: a load past the tile repeats the thread's first unit, and
@@ -298,7 +312,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 				stage[ldsPos[k]] = buf[k];
 			RPWA_STAGE_FETCH(min(tile + step, lastTile));
 			lds_barrier();
-			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gSub, gSec, G, lnS, lnC,
+			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gSub, gSec, G, mant, expo,
 			                         wsum);
 		}
 	}
@@ -324,8 +338,8 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		}
 	}
 	if (wid == 0 && (E == 64 || lane < E)) {
-		scal[lane * 3 + 0] = lnS;
-		scal[lane * 3 + 1] = -lnC;  // Kahan keeps c = (rounded - exact); value = s - c
+		scal[lane * 3 + 0] = -(log(mant) + (double)expo * 0.69314718055994530942);
+		scal[lane * 3 + 1] = 0.;
 		scal[lane * 3 + 2] = wsum;
 	}
 	lds_barrier();
