@@ -16,15 +16,6 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask)
 	return __shfl_xor(v, mask, 64);
 }
 
-// compensated accumulation of one term (utilities/sumAccumulators.hpp:166-177)
-__device__ __forceinline__ void kahan_add(double& s, double& c, double x)
-{
-	const double y = x - c;
-	const double t = s + y;
-	c = (t - s) - y;
-	s = t;
-}
-
 // Neumaier step: (s, c) += x with the rounding error of the addition kept in c
 __device__ __forceinline__ void neumaier_add(double& s, double& c, double x)
 {
@@ -167,7 +158,8 @@ __global__ __launch_bounds__(kWgThreads) void event_kernel(EventArgs a)
 				G[r][k][p] = make_double2(0., 0.);
 			}
 
-	double lnS = 0., lnC = 0.;  // compensated sum of -ln sigma_e (group leaders)
+	double mant = 1.;  // running product of sigma_e as mant 2^expo, mant in [1, 2) (group leaders);
+	int expo = 0;      // its log is taken once at the end instead of one log per event
 	double wsum = 0.;           // sum of -2/sigma_e (group leaders)
 	const uint64_t nWavesGrid = (uint64_t)gridDim.x * kWavesPerWg;
 	const uint64_t rowBytes = (uint64_t)a.rowLen * BPC;
@@ -230,9 +222,19 @@ __global__ __launch_bounds__(kWgThreads) void event_kernel(EventArgs a)
 		sigma += a.flat2;
 		double w = 0.;
 		if (ev) {
-			if (lig == 0)
-				kahan_add(lnS, lnC, -log(sigma));
-			w = -2. / sigma;
+			if (lig == 0) {
+				mant *= sigma;
+				const uint64_t ub = (uint64_t)__double_as_longlong(mant);
+				const int ex = (int)((ub >> 52) & 0x7FF);
+				if (ex != 0 && ex != 0x7FF) {
+					expo += ex - 1023;
+					mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+				}
+			}
+			// -2/sigma from the hardware reciprocal and one Newton step
+			double r = __builtin_amdgcn_rcp(sigma);
+			r = fma(fma(-sigma, r, 1.), r, r);
+			w = -2. * r;
 			if (lig == 0)
 				wsum += w;
 		}
@@ -263,8 +265,8 @@ __global__ __launch_bounds__(kWgThreads) void event_kernel(EventArgs a)
 				}
 	__shared__ double scal[kWavesPerWg][GPW][3];
 	if (lig == 0) {
-		scal[wid][grp][0] = lnS;
-		scal[wid][grp][1] = -lnC;  // Kahan keeps c = (rounded - exact); value = s - c
+		scal[wid][grp][0] = -(log(mant) + (double)expo * 0.69314718055994530942);
+		scal[wid][grp][1] = 0.;
 		scal[wid][grp][2] = wsum;
 	}
 	// waves add their gradient sums into LDS in fixed order 0,1,2,3

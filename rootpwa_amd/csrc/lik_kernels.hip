@@ -3,7 +3,7 @@
 // One pass over the decay amplitudes computes, for every event e,
 //   S_r,s(e) = sum_i T_r,s,i A_s,i(e)            (sector s = reflectivity)
 //   sigma_e  = sum_r sum_s |S_r,s(e)|^2 + flat^2
-// and accumulates  sum_e -ln sigma_e  (compensated) and the event part of the
+// and accumulates  sum_e -ln sigma_e  and the event part of the
 // gradient  G_r,s,i = sum_e (-2/sigma_e) S_r,s(e) conj(A_s,i(e)),  i.e. the
 // quantities of pwaLikelihood::FdF's event loop (partialWaveFit/pwaLikelihood.cc:213-244)
 // and of cuda/likelihoodKernel.cu:84-259, fused into one kernel.

@@ -1,7 +1,7 @@
 // stage_kernel.hip -- the LDS-staged event kernel: one lane per event.
 //
 // Computes exactly what the register-tile kernels of lik_kernels.hip compute
-// (sum_e -ln sigma_e compensated, sum_e -2/sigma_e and the event part of the
+// (sum_e -ln sigma_e, sum_e -2/sigma_e and the event part of the
 // gradient, pwaLikelihood.cc:213-244), with a different work decomposition
 // that keeps neither the production amplitudes nor a slice of the amplitude
 // row in the registers of every lane:
