@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		const uint32_t row = g / a.rowUnits;
 		const bool in = g < a.tileUnits;
 		ldsPos[k] = in ? row * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
-		ldOff[k] = in ? g : (uint32_t)tid;
+		ldOff[k] = (in ? g : (uint32_t)tid) * 16u;  // byte offset in the tile
 	}
 
 	// G-phase role: unit group gi of sector gSec, units [gU0, min(gU0 + UG, gEnd))
@@ -290,12 +290,21 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 	// block's last tile re-reads the array's last tile, so that every stage
 	// write waits for the same loads).
 	f32x4 buf[kMaxLoads];
+	// Tiles before the array's last are complete: their loads are a scalar tile
+	// base plus the thread's fixed byte offsets; the last tile clamps its indices.
 #define RPWA_STAGE_FETCH(T_)                                                           \
 	do {                                                                               \
-		const uint64_t base_ = (uint64_t)(T_) * a.tileUnits;                          \
-		_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k) {                        \
-			const uint64_t g_ = base_ + ldOff[k];                                      \
-			buf[k] = a.amps[g_ < a.lastUnit ? g_ : a.lastUnit];                        \
+		const uint32_t t_ = (T_);                                                      \
+		if (t_ < lastTile) {                                                           \
+			const char* tb_ = reinterpret_cast<const char*>(a.amps + (uint64_t)t_ * a.tileUnits); \
+			_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k)                      \
+				buf[k] = *reinterpret_cast<const f32x4*>(tb_ + ldOff[k]);              \
+		} else {                                                                       \
+			const uint64_t base_ = (uint64_t)t_ * a.tileUnits;                        \
+			_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k) {                    \
+				const uint64_t g_ = base_ + ldOff[k] / 16u;                            \
+				buf[k] = a.amps[g_ < a.lastUnit ? g_ : a.lastUnit];                    \
+			}                                                                          \
 		}                                                                              \
 	} while (0)
 
