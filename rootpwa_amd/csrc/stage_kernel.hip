@@ -209,20 +209,21 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 
 	// ---- G phase: thread = (group of UG units of one sector, event subset) ----
 	if (gOn) {
-		const double2* up = part + gSec * R;
-		uint32_t cu[UG];  // unit of each group slot (clamped into the row; slots past the sector are discarded)
-#pragma unroll
-		for (int j = 0; j < UG; ++j)
-			cu[j] = min(gU0 + (uint32_t)j, a.rowUnits - 1);
-		for (uint32_t e = gSub; e < (uint32_t)E; e += a.gSubs) {
-			const f32x4* row = stage + e * a.rowUnitsP;
+		// The slots of a group are consecutive units, read at fixed offsets from one
+		// pointer per event.  Slots past the sector (or the row) read the next units,
+		// the next row or the LDS areas after the stage, all inside the allocation;
+		// their sums are discarded.
+		const f32x4* ap = stage + gSub * a.rowUnitsP + gU0;
+		const double2* up = part + gSub * PS + gSec * R;
+		const uint32_t aStep = a.gSubs * a.rowUnitsP, uStep = a.gSubs * PS;
+		for (uint32_t e = gSub; e < (uint32_t)E; e += a.gSubs, ap += aStep, up += uStep) {
 			double2 u[R];
 #pragma unroll
 			for (int r = 0; r < R; ++r)
-				u[r] = up[e * PS + r];
+				u[r] = up[r];
 #pragma unroll
 			for (int j = 0; j < UG; ++j) {
-				const f32x4 v = row[cu[j]];
+				const f32x4 v = ap[j];
 #pragma unroll
 				for (int k = 0; k < P; ++k) {
 					const double2 x = unit_cplx<MIXED>(v, k);
