@@ -62,8 +62,31 @@ int rpwa_pwalik_init(rpwa_pwalik* L, uint32_t nmbWaves, const char* const* names
                      const double* thresholds, uint32_t rank, double massBinCenter);
 int rpwa_pwalik_use_normalized_amps(rpwa_pwalik* L, int flag);
 int rpwa_pwalik_set_prior(rpwa_pwalik* L, int halfCauchy, double cauchyWidth);
+/* normalizedAmpsUsed, priorType (1 = HALF_CAUCHY), cauchyWidth, cudaEnabled
+ * (partialWaveFit/pwaLikelihood.h:208-215); any output may be NULL */
+int rpwa_pwalik_settings(const rpwa_pwalik* L, int* useNormalized, int* halfCauchy, double* cauchyWidth,
+                         int* gpuEnabled);
+/* enableCuda (partialWaveFit/pwaLikelihood.h:208): the GPU is the only
+ * event-sum backend, so with flag = 0 every evaluation fails with an error */
+int rpwa_pwalik_enable_gpu(rpwa_pwalik* L, int flag);
+/* setQuiet (partialWaveFit/pwaLikelihood.h:154), for every likelihood */
+int rpwa_pwalik_set_quiet(int flag);
 int rpwa_pwalik_add_norm_integral(rpwa_pwalik* L, const rpwa_intmat* m);
 int rpwa_pwalik_add_acc_integral(rpwa_pwalik* L, rpwa_intmat* m, uint32_t accEventsOverride);
+/* setOnTheFlyBinning (partialWaveFit/pwaLikelihood.h:230-231, .cc:1391-1436,
+ * called from pyInterface/module/_likelihood.py:52): keep the entries with
+ * lower[k] <= values[k][e] < upper[k] for every binning variable k; values is
+ * [nmbVars][nmbEntries] over all entries of the event files, concatenated in
+ * the order of the amplitude arrays (the variables' names are for messages and
+ * must be distinct).  nmbVars = 0 keeps every entry.  After this call
+ * add_amplitude takes arrays of nmbEntries amplitudes and keeps the selected
+ * ones in entry order; add_amplitudes_device is refused.  Call before the
+ * first add_amplitude. */
+int rpwa_pwalik_set_on_the_fly_binning(rpwa_pwalik* L, uint32_t nmbVars, const char* const* varNames,
+                                       const double* lower, const double* upper, uint64_t nmbEntries,
+                                       const double* values);
+/* number of entries that the on-the-fly binning selected */
+int rpwa_pwalik_nmb_entries_in_bin(const rpwa_pwalik* L, uint64_t* nmbEntries);
 int rpwa_pwalik_add_amplitude(rpwa_pwalik* L, const char* waveName, const double* amps, uint64_t nmbEvents);
 /* all waves at once from device memory, [N][W] complex128: the refl -1 waves,
  * then the refl +1 waves, each group in init() wave-list order */
@@ -84,10 +107,24 @@ int rpwa_pwalik_doderivative(const rpwa_pwalik* L, const double* par, uint32_t k
 /* nmbPars x nmbPars row-major */
 int rpwa_pwalik_hessian(const rpwa_pwalik* L, const double* par, double* hessian);
 int rpwa_pwalik_covariance(const rpwa_pwalik* L, const double* par, double* covariance);
+/* CovarianceMatrix(hessian) (partialWaveFit/pwaLikelihood.h:185, .cc:927-979):
+ * inverse of the P x P Hessian reduced to the free parameters, zero rows and
+ * columns at fixed parameters */
+int rpwa_pwalik_covariance_from_hessian(const rpwa_pwalik* L, const double* hessian, double* covariance);
 int rpwa_pwalik_correct_param_signs(const rpwa_pwalik* L, const double* par, double* out);
 /* [rank][2][nmbWavesReflMax] complex and the flat amplitude */
 int rpwa_pwalik_copy_from_par(const rpwa_pwalik* L, const double* par, double* prodAmps, double* flat);
 int rpwa_pwalik_copy_to_par(const rpwa_pwalik* L, const double* prodAmps, double flat, double* par);
+/* buildProdAmpArrays (partialWaveFit/pwaLikelihood.h:239-243, .cc:1624-1661):
+ * *n production amplitudes (complex, at most nmbPars), their parameter indices
+ * as (re, im) pairs with -1 for none, and their names joined by newlines into
+ * names[namesLen]; prodAmps, parIndices and names may be NULL */
+int rpwa_pwalik_build_prod_amp_arrays(const rpwa_pwalik* L, const double* par, int withFlat, double* prodAmps,
+                                      int32_t* parIndices, uint32_t* n, char* names, uint64_t namesLen);
+/* print() (funcInfo = 0) or printFuncInfo() (funcInfo = 1) as text
+ * (partialWaveFit/pwaLikelihood.h:245-246); *needed = bytes with the
+ * terminating zero; text may be NULL to ask for the size */
+int rpwa_pwalik_print(const rpwa_pwalik* L, int funcInfo, char* text, uint64_t textLen, uint64_t* needed);
 /* integralMatrices(withFlat): (W[+1])^2 complex each, phase-space integrals W[+1] */
 int rpwa_pwalik_integral_matrices(const rpwa_pwalik* L, int withFlat, double* normMatrix, double* accMatrix,
                                   double* phaseSpaceIntegral);

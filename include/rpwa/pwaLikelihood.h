@@ -160,6 +160,17 @@ public:
 	          const double massBinCenter = 0.);
 	bool addNormIntegral(const ampIntegralMatrix& normMatrix);
 	bool addAccIntegral(ampIntegralMatrix& accMatrix, const unsigned int accEventsOverride = 0);
+	// lower and upper edge of a bin in each binning variable
+	typedef std::map<std::string, std::pair<double, double> > multibinBoundariesType;
+	// on-the-fly binning (pwaLikelihood.cc:1391-1436): keep the events whose
+	// binning variables v satisfy lower <= v < upper for every variable of the
+	// bin.  eventVariables holds one array per variable over all entries of the
+	// event files, concatenated in the order of the amplitude arrays; after this
+	// call addAmplitude() takes arrays over all those entries and keeps the
+	// selected ones in entry order.  Call before the first addAmplitude().
+	bool setOnTheFlyBinning(const multibinBoundariesType& multibinBoundaries,
+	                        const std::map<std::string, std::vector<double> >& eventVariables);
+	unsigned long nmbEntriesInBin() const { return _entriesInBin.size(); }
 	// amplitudes of one wave for all events (host memory)
 	bool addAmplitude(const std::string& waveName, const std::complex<double>* amps, const unsigned long nmbEvents);
 	// amplitudes of all waves, already in device memory on the GPU device, as
@@ -237,6 +248,10 @@ private:
 	std::vector<fitParameter> _parameters;
 	std::map<std::string, std::pair<unsigned int, unsigned int> > _waveParams;
 	std::vector<std::pair<int, int> > _prodAmpToFuncParMap;  // [rank][2][Wmax]
+
+	bool _onTheFlyBinning;
+	unsigned long _nmbEntriesTotal;             // entries of the event files (on-the-fly binning)
+	std::vector<unsigned long> _entriesInBin;   // selected entries, ascending
 
 	std::vector<std::complex<double> > _decayAmps[2];  // host copy [N][W_refl] (addAmplitude path)
 	const double* _deviceAmps;                         // addAmplitudesDevice path (not owned)

@@ -115,7 +115,7 @@ pwaLikelihood<complexT>::pwaLikelihood()
 	: _nmbEvents(0), _rank(1), _nmbWaves(0), _nmbWavesReflMax(0), _nmbPars(0), _nmbParsFixed(0),
 	  _initialized(false), _normIntAdded(false), _accIntAdded(false), _initFinished(false), _gpuEnabled(true),
 	  _device(0), _useNormalizedAmps(true), _priorType(FLAT), _cauchyWidth(0.5), _numbAccEvents(0), _totAcc(0),
-	  _deviceAmps(nullptr), _evalCacheValid(false)
+	  _onTheFlyBinning(false), _nmbEntriesTotal(0), _deviceAmps(nullptr), _evalCacheValid(false)
 {
 	_nmbWavesRefl[0] = 0;
 	_nmbWavesRefl[1] = 0;
@@ -129,7 +129,9 @@ pwaLikelihood<complexT>::pwaLikelihood(const pwaLikelihood& o)
 	  _initFinished(o._initFinished), _gpuEnabled(o._gpuEnabled), _device(o._device),
 	  _useNormalizedAmps(o._useNormalizedAmps), _priorType(o._priorType), _cauchyWidth(o._cauchyWidth),
 	  _numbAccEvents(o._numbAccEvents), _totAcc(o._totAcc), _parameters(o._parameters),
-	  _waveParams(o._waveParams), _prodAmpToFuncParMap(o._prodAmpToFuncParMap), _deviceAmps(o._deviceAmps),
+	  _waveParams(o._waveParams), _prodAmpToFuncParMap(o._prodAmpToFuncParMap),
+	  _onTheFlyBinning(o._onTheFlyBinning), _nmbEntriesTotal(o._nmbEntriesTotal), _entriesInBin(o._entriesInBin),
+	  _deviceAmps(o._deviceAmps),
 	  _parCache(o._parCache), _derivCache(o._derivCache), _evalCacheValid(false), _normMatrix(o._normMatrix),
 	  _accMatrix(o._accMatrix), _phaseSpaceIntegral(o._phaseSpaceIntegral), _ctx(o._ctx)
 {
@@ -329,10 +331,70 @@ bool pwaLikelihood<complexT>::addAccIntegral(ampIntegralMatrix& accMatrix, const
 	return true;
 }
 
+// setOnTheFlyBinning (pwaLikelihood.cc:1391-1436): the entries of the event
+// files that lie in the bin, as half-open intervals [lower, upper)
+template <typename complexT>
+bool pwaLikelihood<complexT>::setOnTheFlyBinning(const multibinBoundariesType& multibinBoundaries,
+                                                 const std::map<std::string, std::vector<double> >& eventVariables)
+{
+	if (_nmbEvents != 0 or _deviceAmps) {
+		std::cerr << "pwaLikelihood: setOnTheFlyBinning() must come before the amplitudes are added" << std::endl;
+		return false;
+	}
+	if (eventVariables.empty()) {
+		std::cerr << "pwaLikelihood: no event variables given for on-the-fly binning" << std::endl;
+		return false;
+	}
+	const unsigned long nmbEntries = eventVariables.begin()->second.size();
+	for (const auto& v : eventVariables)
+		if (v.second.size() != nmbEntries) {
+			std::cerr << "pwaLikelihood: event variable '" << v.first << "' has " << v.second.size()
+			          << " entries, expected " << nmbEntries << std::endl;
+			return false;
+		}
+	std::vector<const double*> values;
+	std::vector<double> lower, upper;
+	for (const auto& b : multibinBoundaries) {
+		const auto it = eventVariables.find(b.first);
+		if (it == eventVariables.end()) {
+			std::cerr << "pwaLikelihood: binning variable '" << b.first << "' is not among the event variables"
+			          << std::endl;
+			return false;
+		}
+		if (_debug)
+			std::cout << "pwaLikelihood: use bin in '" << b.first << "' with range [" << b.second.first << ", "
+			          << b.second.second << ")" << std::endl;
+		values.push_back(it->second.data());
+		lower.push_back(b.second.first);
+		upper.push_back(b.second.second);
+	}
+	std::vector<unsigned long> entries;
+	for (unsigned long e = 0; e < nmbEntries; ++e) {
+		bool useEvent = true;
+		for (size_t k = 0; k < values.size(); ++k) {
+			const double v = values[k][e];
+			// a NaN compares false both ways and is kept, as in the reference
+			if (v < lower[k] or v >= upper[k]) {
+				useEvent = false;
+				break;
+			}
+		}
+		if (useEvent)
+			entries.push_back(e);
+	}
+	if (_debug)
+		std::cout << "pwaLikelihood: found " << entries.size() << " of " << nmbEntries
+		          << " entries to be in the given bin" << std::endl;
+	_onTheFlyBinning = true;
+	_nmbEntriesTotal = nmbEntries;
+	_entriesInBin.swap(entries);
+	return true;
+}
+
 // addAmplitude (pwaLikelihood.cc:1112-1288) for one wave's amplitude array
 template <typename complexT>
 bool pwaLikelihood<complexT>::addAmplitude(const std::string& waveName, const std::complex<double>* amps,
-                                           const unsigned long nmbEvents)
+                                           const unsigned long nmbEntries)
 {
 	if (not _accIntAdded) {
 		std::cerr << "pwaLikelihood: call addAccIntegral() before addAmplitude()" << std::endl;
@@ -342,6 +404,12 @@ bool pwaLikelihood<complexT>::addAmplitude(const std::string& waveName, const st
 		std::cerr << "pwaLikelihood: amplitudes were already given in device memory" << std::endl;
 		return false;
 	}
+	if (_onTheFlyBinning and nmbEntries != _nmbEntriesTotal) {
+		std::cerr << "pwaLikelihood: amplitude array of wave '" << waveName << "' has " << nmbEntries
+		          << " entries, the event variables of the on-the-fly binning have " << _nmbEntriesTotal << std::endl;
+		return false;
+	}
+	const unsigned long nmbEvents = _onTheFlyBinning ? _entriesInBin.size() : nmbEntries;
 	if (not amps or nmbEvents == 0) {
 		std::cerr << "pwaLikelihood: no events to load" << std::endl;
 		return false;
@@ -366,7 +434,7 @@ bool pwaLikelihood<complexT>::addAmplitude(const std::string& waveName, const st
 	bool fixPar = (_accMatrix[matIndex(refl, waveIndex, refl, waveIndex)] == 0.);
 	const unsigned int W = _nmbWavesRefl[refl];
 	for (unsigned long iEvt = 0; iEvt < _nmbEvents; ++iEvt) {
-		std::complex<double> a = amps[iEvt];
+		std::complex<double> a = amps[_onTheFlyBinning ? _entriesInBin[iEvt] : iEvt];
 		if (_useNormalizedAmps) {
 			if (normInt == 0. and a != 0.) {
 				std::cerr << "pwaLikelihood: normalization integral for wave '" << waveName
@@ -406,6 +474,11 @@ bool pwaLikelihood<complexT>::addAmplitudesDevice(const double* d_amps, const un
 	}
 	if (_nmbEvents != 0) {
 		std::cerr << "pwaLikelihood: amplitudes were already added" << std::endl;
+		return false;
+	}
+	if (_onTheFlyBinning) {
+		std::cerr << "pwaLikelihood: on-the-fly binning selects from host amplitude arrays; device amplitudes "
+		          << "must hold the events of the bin only" << std::endl;
 		return false;
 	}
 	if (_useNormalizedAmps)

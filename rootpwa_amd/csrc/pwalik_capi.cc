@@ -2,7 +2,9 @@
 // (declared in include/rootpwa_amd_pwalik.h).
 #include <complex>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -618,6 +620,156 @@ extern "C" int rpwa_pwalik_integral_matrices(const rpwa_pwalik* L, int withFlat,
 		if (phaseSpaceIntegral)
 			std::memcpy(phaseSpaceIntegral, ps.data(), ps.size() * sizeof(double));
 	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_set_quiet(int flag)
+{
+	clear_error();
+	rpwa::pwaLikelihood<std::complex<double> >::setQuiet(flag != 0);
+	rpwa::pwaLikelihood<std::complex<float> >::setQuiet(flag != 0);
+	return RPWA_OK;
+}
+
+extern "C" int rpwa_pwalik_enable_gpu(rpwa_pwalik* L, int flag)
+{
+	GUARD_BEGIN
+	if (!L)
+		return fail("rpwa_pwalik_enable_gpu", "null pointer");
+	WITH_LIK(L, lik.enableCuda(flag != 0));
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_settings(const rpwa_pwalik* L, int* useNormalized, int* halfCauchy, double* cauchyWidth,
+                                    int* gpuEnabled)
+{
+	GUARD_BEGIN
+	if (!L)
+		return fail("rpwa_pwalik_settings", "null pointer");
+	WITH_LIK(L, {
+		typedef typename std::remove_reference<decltype(lik)>::type likT;
+		if (useNormalized)
+			*useNormalized = lik.normalizedAmpsUsed() ? 1 : 0;
+		if (halfCauchy)
+			*halfCauchy = lik.priorType() == likT::HALF_CAUCHY ? 1 : 0;
+		if (cauchyWidth)
+			*cauchyWidth = lik.cauchyWidth();
+		if (gpuEnabled)
+			*gpuEnabled = lik.cudaEnabled() ? 1 : 0;
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_set_on_the_fly_binning(rpwa_pwalik* L, uint32_t nmbVars, const char* const* varNames,
+                                                  const double* lower, const double* upper, uint64_t nmbEntries,
+                                                  const double* values)
+{
+	GUARD_BEGIN
+	if (!L || (nmbVars > 0 && (!varNames || !lower || !upper || !values)))
+		return fail("rpwa_pwalik_set_on_the_fly_binning", "null pointer");
+	std::map<std::string, std::pair<double, double> > bounds;
+	std::map<std::string, std::vector<double> > vars;
+	for (uint32_t k = 0; k < nmbVars; ++k) {
+		if (!varNames[k])
+			return fail("rpwa_pwalik_set_on_the_fly_binning", "null variable name");
+		const std::string name(varNames[k]);
+		if (bounds.count(name))
+			return fail("rpwa_pwalik_set_on_the_fly_binning", "binning variable given twice");
+		bounds[name] = std::make_pair(lower[k], upper[k]);
+		vars[name].assign(values + (size_t)k * nmbEntries, values + (size_t)(k + 1) * nmbEntries);
+	}
+	if (nmbVars == 0)  // an empty bin definition keeps every entry
+		vars[""].assign((size_t)nmbEntries, 0.);
+	bool ok = false;
+	WITH_LIK(L, ok = lik.setOnTheFlyBinning(bounds, vars));
+	if (!ok)
+		return fail("rpwa_pwalik_set_on_the_fly_binning", "rejected (amplitudes already added?)");
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_nmb_entries_in_bin(const rpwa_pwalik* L, uint64_t* nmbEntries)
+{
+	GUARD_BEGIN
+	if (!L || !nmbEntries)
+		return fail("rpwa_pwalik_nmb_entries_in_bin", "null pointer");
+	WITH_LIK(L, *nmbEntries = lik.nmbEntriesInBin());
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_covariance_from_hessian(const rpwa_pwalik* L, const double* hessian, double* covariance)
+{
+	GUARD_BEGIN
+	if (!L || !hessian || !covariance)
+		return fail("rpwa_pwalik_covariance_from_hessian", "null pointer");
+	WITH_LIK(L, {
+		const unsigned int P = lik.nmbPars();
+		rpwa::denseMatrix h(P);
+		std::memcpy(h.data.data(), hessian, (size_t)P * P * sizeof(double));
+		const rpwa::denseMatrix C = lik.CovarianceMatrix(h);
+		std::memcpy(covariance, C.data.data(), C.data.size() * sizeof(double));
+	});
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_build_prod_amp_arrays(const rpwa_pwalik* L, const double* par, int withFlat,
+                                                 double* prodAmps, int32_t* parIndices, uint32_t* n, char* names,
+                                                 uint64_t namesLen)
+{
+	GUARD_BEGIN
+	if (!L || !par || !n)
+		return fail("rpwa_pwalik_build_prod_amp_arrays", "null pointer");
+	std::vector<std::complex<double> > amps;
+	std::vector<std::pair<int, int> > idx;
+	std::vector<std::string> nm;
+	WITH_LIK(L, lik.buildProdAmpArrays(par, amps, idx, nm, withFlat != 0));
+	std::string joined;
+	for (size_t k = 0; k < nm.size(); ++k)
+		joined += (k ? "\n" : "") + nm[k];
+	if (names && joined.size() + 1 > namesLen)
+		return fail("rpwa_pwalik_build_prod_amp_arrays", "name buffer too small");
+	*n = (uint32_t)amps.size();
+	for (size_t k = 0; k < amps.size(); ++k) {
+		if (prodAmps) {
+			prodAmps[2 * k] = amps[k].real();
+			prodAmps[2 * k + 1] = amps[k].imag();
+		}
+		if (parIndices) {
+			parIndices[2 * k] = idx[k].first;
+			parIndices[2 * k + 1] = idx[k].second;
+		}
+	}
+	if (names)
+		std::memcpy(names, joined.c_str(), joined.size() + 1);
+	return RPWA_OK;
+	GUARD_END
+}
+
+extern "C" int rpwa_pwalik_print(const rpwa_pwalik* L, int funcInfo, char* text, uint64_t textLen, uint64_t* needed)
+{
+	GUARD_BEGIN
+	if (!L)
+		return fail("rpwa_pwalik_print", "null pointer");
+	std::ostringstream out;
+	WITH_LIK(L, {
+		if (funcInfo)
+			lik.printFuncInfo(out);
+		else
+			lik.print(out);
+	});
+	const std::string s = out.str();
+	if (needed)
+		*needed = s.size() + 1;
+	if (text) {
+		if (textLen < s.size() + 1)
+			return fail("rpwa_pwalik_print", "text buffer too small");
+		std::memcpy(text, s.c_str(), s.size() + 1);
+	}
 	return RPWA_OK;
 	GUARD_END
 }

@@ -65,6 +65,15 @@ PROTOTYPES = {
     "rpwa_pwalik_call_info": (_i, [_vp, _u32p, _dp]),
     "rpwa_pwalik_gpu_context": (_i, [_vp, ctypes.POINTER(_vp)]),
     "rpwa_pwalik_hessian_eigen": (_i, [_vp, _dp, _dp, _dp]),
+    "rpwa_pwalik_set_quiet": (_i, [_i]),
+    "rpwa_pwalik_enable_gpu": (_i, [_vp, _i]),
+    "rpwa_pwalik_settings": (_i, [_vp, _ip, _ip, _dp, _ip]),
+    "rpwa_pwalik_set_on_the_fly_binning": (_i, [_vp, _u32, _cpp, _dp, _dp, _u64, _dp]),
+    "rpwa_pwalik_nmb_entries_in_bin": (_i, [_vp, _u64p]),
+    "rpwa_pwalik_covariance_from_hessian": (_i, [_vp, _dp, _dp]),
+    "rpwa_pwalik_build_prod_amp_arrays": (_i, [_vp, _dp, _i, _dp, ctypes.POINTER(ctypes.c_int32), _u32p,
+                                               ctypes.c_char_p, _u64]),
+    "rpwa_pwalik_print": (_i, [_vp, _i, ctypes.c_char_p, _u64, _u64p]),
     "rpwa_fit_start_values": (_i, [_vp, _u32, _dp]),
     "rpwa_fit_nlopt": (_i, [_vp, _u32, _i, _vp, _dp, _dp, _dp, _vp]),
     "rpwa_fit_minuit": (_i, [_vp, _u32, _dp, _i, _vp, _dp, _dp, _dp, _vp]),
@@ -201,11 +210,54 @@ class ampIntegralMatrix:
         _call("rpwa_intmat_read_ascii", self.h, fileName.encode())
 
 
+class fitParameter:
+    """pwaLikelihood::fitParameter (partialWaveFit/pwaLikelihood.h:113-148)."""
+
+    def __init__(self, waveName, rank, threshold, fixed, realPart):
+        self._waveName, self._rank, self._threshold = waveName, int(rank), float(threshold)
+        self._fixed, self._realPart = bool(fixed), bool(realPart)
+
+    def waveName(self):
+        return self._waveName
+
+    def rank(self):
+        return self._rank
+
+    def threshold(self):
+        return self._threshold
+
+    def fixed(self):
+        return self._fixed
+
+    def realPart(self):
+        return self._realPart
+
+    def parName(self):
+        """V<rank>_<wave>_RE / _IM, V_flat_RE for the flat wave (pwaLikelihood.cc:107-122)."""
+        rank = "" if self._waveName == "flat" else str(self._rank)
+        return "V%s_%s_%s" % (rank, self._waveName, "RE" if self._realPart else "IM")
+
+    def __eq__(self, other):
+        return isinstance(other, fitParameter) and (self._waveName, self._rank, self._threshold, self._fixed,
+                                                    self._realPart) == (other._waveName, other._rank,
+                                                                        other._threshold, other._fixed,
+                                                                        other._realPart)
+
+    def __repr__(self):
+        return "fitParameter(%s, fixed=%s)" % (self.parName(), self._fixed)
+
+
 class pwaLikelihood:
     """rpwa::pwaLikelihood evaluated on the GPU (C++ object behind the C ABI)."""
 
     FLAT = 0
     HALF_CAUCHY = 1
+    # functionCallEnum (partialWaveFit/pwaLikelihood.h:77-84)
+    FDF, GRADIENT, DOEVAL, DODERIVATIVE, HESSIAN = range(5)
+
+    @staticmethod
+    def setQuiet(flag=True):
+        _call("rpwa_pwalik_set_quiet", int(bool(flag)))
 
     def __init__(self, precision=capi.PREC_FP64, _handle=None):
         self.h = _vp()
@@ -248,8 +300,66 @@ class pwaLikelihood:
     def useNormalizedAmps(self, flag=True):
         _call("rpwa_pwalik_use_normalized_amps", self.h, int(bool(flag)))
 
-    def setPriorType(self, prior, cauchyWidth=0.5):
-        _call("rpwa_pwalik_set_prior", self.h, int(prior == self.HALF_CAUCHY), float(cauchyWidth))
+    def _settings(self):
+        un, hc, gpu = _i(), _i(), _i()
+        w = ctypes.c_double()
+        _call("rpwa_pwalik_settings", self.h, ctypes.byref(un), ctypes.byref(hc), ctypes.byref(w), ctypes.byref(gpu))
+        return bool(un.value), (self.HALF_CAUCHY if hc.value else self.FLAT), w.value, bool(gpu.value)
+
+    def normalizedAmpsUsed(self):
+        return self._settings()[0]
+
+    def setPriorType(self, prior=0, cauchyWidth=None):
+        """setPriorType(priorType) keeps the Cauchy width; giving cauchyWidth also sets it."""
+        width = self.cauchyWidth() if cauchyWidth is None else float(cauchyWidth)
+        _call("rpwa_pwalik_set_prior", self.h, int(prior == self.HALF_CAUCHY), width)
+
+    def priorType(self):
+        return self._settings()[1]
+
+    def setCauchyWidth(self, cauchyWidth):
+        _call("rpwa_pwalik_set_prior", self.h, int(self.priorType() == self.HALF_CAUCHY), float(cauchyWidth))
+
+    def cauchyWidth(self):
+        return self._settings()[2]
+
+    def enableCuda(self, flag=True):
+        """The GPU is the only event-sum backend: with flag False every evaluation raises."""
+        _call("rpwa_pwalik_enable_gpu", self.h, int(bool(flag)))
+
+    def cudaEnabled(self):
+        return self._settings()[3]
+
+    def setOnTheFlyBinning(self, multibinBoundaries, eventVariables):
+        """setOnTheFlyBinning (partialWaveFit/pwaLikelihood.cc:1391-1436).
+
+        multibinBoundaries: {variable: (lower, upper)}, an event is in the bin
+        when lower <= value < upper for every variable.  eventVariables:
+        {variable: array over all entries}, or a list of such dicts, one per
+        event file, in the order in which the amplitude arrays concatenate the
+        files.  addAmplitude then takes arrays over all entries."""
+        if isinstance(eventVariables, dict):
+            eventVariables = [eventVariables]
+        names = sorted(multibinBoundaries)
+        for v in names:
+            if any(v not in ev for ev in eventVariables):
+                raise KeyError("binning variable '%s' is not among the event variables" % v)
+        cols = [np.concatenate([np.asarray(ev[v], dtype=np.float64).ravel() for ev in eventVariables])
+                for v in names]
+        nEntries = cols[0].size if cols else sum(
+            len(next(iter(ev.values()))) if ev else 0 for ev in eventVariables)
+        if any(c.size != nEntries for c in cols):
+            raise ValueError("event variables differ in length")
+        values = np.ascontiguousarray(np.stack(cols) if cols else np.zeros((0, nEntries)))
+        lower = np.array([float(multibinBoundaries[v][0]) for v in names])
+        upper = np.array([float(multibinBoundaries[v][1]) for v in names])
+        _call("rpwa_pwalik_set_on_the_fly_binning", self.h, len(names), _names(names), _d(lower), _d(upper),
+              int(nEntries), _d(values))
+
+    def nmbEntriesInBin(self):
+        n = ctypes.c_uint64()
+        _call("rpwa_pwalik_nmb_entries_in_bin", self.h, ctypes.byref(n))
+        return n.value
 
     def addNormIntegral(self, m):
         _call("rpwa_pwalik_add_norm_integral", self.h, m.h)
@@ -296,13 +406,23 @@ class pwaLikelihood:
         return self._rank
 
     def parameter(self, i):
-        """(parName, rank, fixed, realPart)"""
-        name = ctypes.create_string_buffer(256)
+        """fitParameter i (partialWaveFit/pwaLikelihood.h:195)."""
+        name = ctypes.create_string_buffer(1024)
         r = ctypes.c_uint32()
         fx = ctypes.c_int()
         rp = ctypes.c_int()
-        _call("rpwa_pwalik_parameter", self.h, int(i), name, 256, ctypes.byref(r), ctypes.byref(fx), ctypes.byref(rp))
-        return name.value.decode(), r.value, bool(fx.value), bool(rp.value)
+        _call("rpwa_pwalik_parameter", self.h, int(i), name, 1024, ctypes.byref(r), ctypes.byref(fx),
+              ctypes.byref(rp))
+        parName = name.value.decode()
+        if parName == "V_flat_RE":
+            return fitParameter("flat", 0, 0.0, bool(fx.value), True)
+        # V<rank>_<wave>_RE|IM
+        wave = parName[len("V%d_" % r.value):-3]
+        thr = {w[0]: (float(w[2]) if len(w) > 2 else 0.0) for w in (self._waves or [])}.get(wave, 0.0)
+        return fitParameter(wave, r.value, thr, bool(fx.value), bool(rp.value))
+
+    def parameters(self):
+        return [self.parameter(i) for i in range(self.nmbPars())]
 
     # evaluation ------------------------------------------------------------
     def _par(self, par):
@@ -341,10 +461,17 @@ class pwaLikelihood:
         _call("rpwa_pwalik_hessian", self.h, _d(p), _d(H))
         return H
 
-    def CovarianceMatrix(self, par):
-        p = self._par(par)
-        C = np.zeros((p.size, p.size))
-        _call("rpwa_pwalik_covariance", self.h, _d(p), _d(C))
+    def CovarianceMatrix(self, parOrHessian):
+        """CovarianceMatrix(par) or CovarianceMatrix(hessian) (pwaLikelihood_py.cc:270-271)."""
+        a = np.asarray(parOrHessian, dtype=np.float64)
+        P = self.nmbPars()
+        C = np.zeros((P, P))
+        if a.ndim == 2:
+            if a.shape != (P, P):
+                raise ValueError("Hessian must be %d x %d" % (P, P))
+            _call("rpwa_pwalik_covariance_from_hessian", self.h, _d(np.ascontiguousarray(a)), _d(C))
+        else:
+            _call("rpwa_pwalik_covariance", self.h, _d(self._par(a)), _d(C))
         return C
 
     def HessianEigenVectors(self, hessian):
@@ -388,6 +515,42 @@ class pwaLikelihood:
               A.view(np.float64).ctypes.data_as(_dp), _d(ps))
         return N, A, ps
 
+    def getIntegralMatrices(self, withFlat=False):
+        return self.integralMatrices(withFlat)
+
+    def buildProdAmpArrays(self, par, withFlat=False):
+        """(prodAmps, parIndices, prodAmpNames) (partialWaveFit/pwaLikelihood.cc:1624-1661);
+        parIndices holds (re, im) parameter indices, -1 where there is none."""
+        p = self._par(par)
+        cap = self.nmbPars()
+        amps = np.zeros(2 * cap)
+        idx = (ctypes.c_int32 * (2 * cap))()
+        n = ctypes.c_uint32()
+        namesLen = sum(len(w[0]) + 24 for w in (self._waves or [])) * max(self._rank, 1) + 64
+        names = ctypes.create_string_buffer(namesLen)
+        _call("rpwa_pwalik_build_prod_amp_arrays", self.h, _d(p), int(bool(withFlat)), _d(amps), idx,
+              ctypes.byref(n), names, namesLen)
+        k = n.value
+        prod = amps[:2 * k].view(np.complex128).copy()
+        pairs = [(idx[2 * j], idx[2 * j + 1]) for j in range(k)]
+        return prod, pairs, names.value.decode().split("\n") if k else []
+
+    def _text(self, funcInfo):
+        need = ctypes.c_uint64()
+        _call("rpwa_pwalik_print", self.h, int(funcInfo), None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value)
+        _call("rpwa_pwalik_print", self.h, int(funcInfo), buf, need.value, None)
+        return buf.value.decode()
+
+    def printFuncInfo(self):
+        return self._text(1)
+
+    def __str__(self):
+        return self._text(0)
+
+    def ncalls(self, func=0):
+        return self.callInfo()[("FdF", "Gradient", "DoEval", "DoDerivative", "Hessian")[func]][0]
+
     def gpuContext(self):
         """Borrowed capi.LikelihoodContext view of the event-sum context."""
         h = _vp()
@@ -407,18 +570,26 @@ class pwaLikelihood:
 
 
 def initLikelihood(waves, amplitudes, normIntegral, accIntegral=None, rank=1, massBinCenter=0.0,
-                   accEventsOverride=0, precision=capi.PREC_FP64, cauchyWidth=None, device=0):
-    """The setup sequence of pyRootPwa.initLikelihood (pyInterface/module/_likelihood.py:17-71).
+                   accEventsOverride=0, precision=capi.PREC_FP64, cauchyWidth=None, device=0,
+                   useNormalizedAmps=True, multibinBoundaries=None, eventVariables=None):
+    """The setup sequence of pyRootPwa.initLikelihood (pyInterface/module/_likelihood.py:4-71).
 
     waves: [(name, reflectivity, threshold)]; amplitudes: dict name -> complex
-    array of data events; normIntegral / accIntegral: ampIntegralMatrix."""
+    array of data events; normIntegral / accIntegral: ampIntegralMatrix.
+    With eventVariables (see pwaLikelihood.setOnTheFlyBinning) the amplitude
+    arrays cover all entries of the event files and the events inside
+    multibinBoundaries (empty or None: all) are kept."""
     L = pwaLikelihood(precision)
     L.setDevice(device)
-    L.init(waves, rank, massBinCenter)
+    L.useNormalizedAmps(useNormalizedAmps)
     if cauchyWidth is not None:
-        L.setPriorType(pwaLikelihood.HALF_CAUCHY, cauchyWidth)
+        L.setPriorType(pwaLikelihood.HALF_CAUCHY)
+        L.setCauchyWidth(cauchyWidth)
+    L.init(waves, rank, massBinCenter)
     L.addNormIntegral(normIntegral)
     L.addAccIntegral(accIntegral if accIntegral is not None else normIntegral, accEventsOverride)
+    if eventVariables is not None:
+        L.setOnTheFlyBinning(multibinBoundaries or {}, eventVariables)
     for w in waves:
         L.addAmplitude(w[0], amplitudes[w[0]])
     L.finishInit()
@@ -440,7 +611,7 @@ class fitResult:
         self.nmbEvaluations = summary.nmbEvaluations
         self.nmbIterations = summary.nmbIterations
         self.nmbNonPositiveEigenvalues = summary.nmbNonPositiveEigenvalues
-        self.parNames = [L.parameter(i)[0] for i in range(L.nmbPars())]
+        self.parNames = [L.parameter(i).parName() for i in range(L.nmbPars())]
 
     def logLikelihood(self):
         return self._logLikelihood

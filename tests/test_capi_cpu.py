@@ -78,7 +78,7 @@ def test_parameter_names():
     L = PL.pwaLikelihood()
     L.init([("1-(1++)0+rho770_01_pi-", +1, 0.0), ("1-(0-+)0+f0980_00_pi-", +1, 0.0),
             ("1-(1++)1-rho770_01_pi-", -1, 0.0)], 2)
-    names = [L.parameter(i)[0] for i in range(L.nmbPars())]
+    names = [L.parameter(i).parName() for i in range(L.nmbPars())]
     # pwaLikelihood.cc:107-122: V<rank>_<wave>_RE/_IM, V_flat_RE for the flat wave
     assert names == ["V0_1-(1++)1-rho770_01_pi-_RE",
                      "V0_1-(1++)0+rho770_01_pi-_RE", "V0_1-(0-+)0+f0980_00_pi-_RE", "V0_1-(0-+)0+f0980_00_pi-_IM",
@@ -90,7 +90,7 @@ def test_thresholds_fix_parameters():
     from rootpwa_amd import likelihood as PL
     L = PL.pwaLikelihood()
     L.init([("a", -1, 0.0), ("b", -1, 1.0), ("c", +1, 0.0), ("d", +1, 0.0), ("e", +1, 2.0)], 1, massBinCenter=1.5)
-    fixed = [L.parameter(i)[2] for i in range(L.nmbPars())]
+    fixed = [L.parameter(i).fixed() for i in range(L.nmbPars())]
     assert sum(fixed) == 2 and L.nmbParsFixed() == 2
 
 
@@ -177,7 +177,7 @@ def test_start_values_follow_mt19937():
             v = 0.01 + (np.sqrt(float(nEv)) - 0.01) * u[2 * i]
             if u[2 * i + 1] > 0.5:
                 v = -v
-            expect[i] = 0.0 if L.parameter(i)[2] else v
+            expect[i] = 0.0 if L.parameter(i).fixed() else v
         got = PL.startValues(L, seed)
         assert np.array_equal(got, expect), seed
     assert L.nmbParsFixed() == 2 and PL.startValues(L, 5)[0] != 0.0
@@ -192,7 +192,7 @@ def test_hessian_eigenvectors_against_lapack():
     L = PL.pwaLikelihood()
     L.init(waves, 2, massBinCenter=2.0)
     P = L.nmbPars()
-    fixed = np.array([L.parameter(i)[2] for i in range(P)], dtype=bool)
+    fixed = np.array([L.parameter(i).fixed() for i in range(P)], dtype=bool)
     assert fixed.sum() == L.nmbParsFixed() > 0
     rng = np.random.default_rng(5)
     B = rng.normal(size=(P, P))
@@ -239,3 +239,118 @@ def test_batch_errors_without_gpu():
         with pytest.raises(capi.RpwaError) as e:
             B.BatchFit(8, 4)
         assert "no GPU" in str(e.value)
+
+
+def test_settings_and_accessors():
+    """Modifiers and accessors of pwaLikelihood.h:154,195-196,208-215 with the
+    reference's defaults (pwaLikelihood.cc:145-149)."""
+    from rootpwa_amd import capi
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    assert L.normalizedAmpsUsed() and L.priorType() == PL.pwaLikelihood.FLAT and L.cauchyWidth() == 0.5
+    L.useNormalizedAmps(False)
+    L.setPriorType(PL.pwaLikelihood.HALF_CAUCHY)
+    assert L.cauchyWidth() == 0.5          # setPriorType keeps the width
+    L.setCauchyWidth(0.25)
+    assert (L.normalizedAmpsUsed(), L.priorType(), L.cauchyWidth()) == (False, PL.pwaLikelihood.HALF_CAUCHY, 0.25)
+    L.setPriorType(PL.pwaLikelihood.FLAT)
+    assert L.priorType() == PL.pwaLikelihood.FLAT and L.cauchyWidth() == 0.25
+    assert L.cudaEnabled()
+    L.enableCuda(False)
+    assert not L.cudaEnabled()
+    L.enableCuda()
+    PL.pwaLikelihood.setQuiet(True)
+    PL.pwaLikelihood.setQuiet(False)
+    L.init([("a", -1, 0.0), ("b", -1, 1.0), ("c", +1, 0.0), ("d", +1, 2.0)], 2, massBinCenter=1.5)
+    pars = L.parameters()
+    assert len(pars) == L.nmbPars() and pars[0] == L.parameter(0)
+    assert [(p.waveName(), p.rank(), p.threshold(), p.fixed(), p.realPart()) for p in pars] == [
+        ("a", 0, 0.0, False, True), ("b", 0, 1.0, False, True), ("b", 0, 1.0, False, False),
+        ("c", 0, 0.0, False, True), ("d", 0, 2.0, True, True), ("d", 0, 2.0, True, False),
+        ("b", 1, 1.0, False, True), ("d", 1, 2.0, True, True),
+        ("flat", 0, 0.0, False, True)]
+    text = str(L)
+    assert "number of function parameters ........... %d" % L.nmbPars() in text and "V1_d_RE" in text
+    assert L.printFuncInfo() == ""          # no calls yet
+    with pytest.raises(capi.RpwaError):
+        L.CovarianceMatrix(np.zeros(L.nmbPars()))   # evaluation before finishInit
+
+
+def test_build_prod_amp_arrays():
+    """buildProdAmpArrays (pwaLikelihood.cc:1624-1661) against copyFromParArray,
+    whose map the oracle pins, and against the parameter names."""
+    from rootpwa_amd import likelihood as PL
+    nW, rank = (3, 5), 3
+    L = PL.pwaLikelihood()
+    L.init(wave_list(nW), rank)
+    par = np.random.default_rng(4).normal(size=L.nmbPars())
+    prod, idx, names = L.buildProdAmpArrays(par)
+    po, flat = O.OracleLikelihood(nW, rank).copy_from_par(par)
+    expect = [(r, refl, w) for r in range(rank) for refl in (0, 1) for w in range(nW[refl]) if w >= r]
+    assert len(prod) == len(expect) == len(idx) == len(names)
+    waves = wave_list(nW)
+    pnames = [p.parName() for p in L.parameters()]
+    for k, (r, refl, w) in enumerate(expect):
+        assert prod[k] == po[r, refl, w]
+        wave = waves[w if refl == 0 else nW[0] + w][0]
+        assert names[k] == "V%d_%s" % (r, wave)
+        assert pnames[idx[k][0]] == names[k] + "_RE"
+        assert (idx[k][1] == -1) == (w == r)
+        if idx[k][1] != -1:
+            assert pnames[idx[k][1]] == names[k] + "_IM"
+    prodF, idxF, namesF = L.buildProdAmpArrays(par, withFlat=True)
+    assert namesF[-1] == "V_flat" and idxF[-1] == (L.nmbPars() - 1, -1) and prodF[-1] == flat
+    assert np.array_equal(prodF[:-1], prod)
+
+
+def test_covariance_from_hessian():
+    """CovarianceMatrix(hessian) (pwaLikelihood.cc:927-979): the inverse on the
+    free parameters, zero at the fixed ones."""
+    from rootpwa_amd import likelihood as PL
+    L = PL.pwaLikelihood()
+    L.init([("a", -1, 0.0), ("b", -1, 3.0), ("c", +1, 0.0), ("d", +1, 0.0)], 1, massBinCenter=1.0)
+    P = L.nmbPars()
+    fixed = np.array([p.fixed() for p in L.parameters()])
+    assert fixed.sum() == 2
+    rng = np.random.default_rng(5)
+    M = rng.normal(size=(P, P))
+    Hm = M @ M.T + P * np.eye(P)
+    C = L.CovarianceMatrix(Hm)
+    free = np.flatnonzero(~fixed)
+    assert np.allclose(C[np.ix_(free, free)] @ Hm[np.ix_(free, free)], np.eye(free.size), atol=1e-12)
+    assert np.all(C[fixed] == 0) and np.all(C[:, fixed] == 0)
+    with pytest.raises(ValueError):
+        L.CovarianceMatrix(Hm[:-1, :-1])
+
+
+def test_on_the_fly_binning_selection():
+    """setOnTheFlyBinning (pwaLikelihood.cc:1391-1436): an entry is in the bin
+    when lower <= v < upper for every variable; entries of several event files
+    concatenate in file order."""
+    from rootpwa_amd import capi
+    from rootpwa_amd import likelihood as PL
+    m = np.array([0.9, 1.0, 1.05, 1.1, 1.2, np.nan, 1.0])
+    t = np.array([0.2, 0.1, 0.5, 0.1, 0.1, 0.1, 0.3])
+    bounds = {"mass": (1.0, 1.1), "tPrime": (0.1, 0.4)}
+    L = PL.pwaLikelihood()
+    L.init(wave_list((1, 2)), 1)
+    L.setOnTheFlyBinning(bounds, [{"mass": m[:3], "tPrime": t[:3]}, {"mass": m[3:], "tPrime": t[3:]}])
+    # 1.0 (lower edge) kept, 1.1 (upper edge) dropped, t 0.5 out, NaN compares false both ways: kept
+    assert L.nmbEntriesInBin() == 3
+    L2 = PL.pwaLikelihood()
+    L2.init(wave_list((1, 2)), 1)
+    L2.setOnTheFlyBinning({}, {"mass": m})
+    assert L2.nmbEntriesInBin() == m.size
+    with pytest.raises(KeyError):
+        L2.setOnTheFlyBinning({"x": (0, 1)}, {"mass": m})
+    # amplitudes must cover all entries; the amplitudes come after the integrals
+    I = PL.ampIntegralMatrix(["neg00", "pos00", "pos01"])
+    I.setSums(np.eye(3, dtype=np.complex128) * 7, 7)
+    L.addNormIntegral(I)
+    L.addAccIntegral(I)
+    with pytest.raises(capi.RpwaError):
+        L.addAmplitude("pos00", np.ones(3))
+    L.addAmplitude("pos00", np.ones(m.size))
+    assert L.nmbEvents() == 3
+    with pytest.raises(capi.RpwaError):       # binning after the amplitudes
+        L.setOnTheFlyBinning(bounds, {"mass": m, "tPrime": t})

@@ -28,7 +28,7 @@ def oracle_newton(Lo, x, free, steps=8):
 
 def check_minimum(Lo, Lg, res, tol_sigma):
     P = Lg.nmbPars()
-    fixed = np.array([Lg.parameter(i)[2] for i in range(P)], dtype=bool)
+    fixed = np.array([Lg.parameter(i).fixed() for i in range(P)], dtype=bool)
     free = np.flatnonzero(~fixed)
     x = res.fitParameters
     assert np.all(x[fixed] == 0.0)
@@ -121,7 +121,7 @@ def test_fixed_parameters_and_rank2(gpu, rank2_thresholds):
     assert len(res.hessianEigenValues) == Lg.nmbPars() - Lg.nmbParsFixed()
     xs, free = check_minimum(Lo, Lg, res, tol_sigma=0.05)
     P = Lg.nmbPars()
-    fixed = [i for i in range(P) if Lg.parameter(i)[2]]
+    fixed = [i for i in range(P) if Lg.parameter(i).fixed()]
     assert np.all(res.covMatrix[fixed, :] == 0) and np.all(res.covMatrix[:, fixed] == 0)
 
 

@@ -141,7 +141,7 @@ def test_hessian_matches_oracle(gpu, pair):
     assert np.max(np.abs(Hg - Ho)) <= 1e-10 * scale
     # covariance = inverse of the Hessian on the free parameters
     C = Lg.CovarianceMatrix(par)
-    free = [i for i in range(Lo.nPars) if not Lg.parameter(i)[2]]
+    free = [i for i in range(Lo.nPars) if not Lg.parameter(i).fixed()]
     Hf = Ho[np.ix_(free, free)]
     assert np.allclose(C[np.ix_(free, free)] @ Hf, np.eye(len(free)), atol=1e-6)
 
@@ -152,3 +152,35 @@ def test_hessian_mixed_precision(gpu):
     Hg = Lg.Hessian(par)
     Ho = Lo.hessian(par)
     assert np.max(np.abs(Hg - Ho)) <= 1e-5 * np.max(np.abs(Ho))
+
+
+def test_on_the_fly_binning_matches_prefiltered(gpu):
+    """A likelihood that selects its events with setOnTheFlyBinning
+    (pwaLikelihood.cc:1143-1215,1391-1436) is the likelihood of the selected
+    events: identical values and gradients, and the oracle's on them."""
+    from rootpwa_amd import likelihood as PL
+    nW, rank = (3, 7), 2
+    ds = H.dataset(nW, rank, 1200, 3000, 77)
+    rng = np.random.default_rng(78)
+    mass = rng.uniform(0.5, 2.5, size=ds["data"].shape[0])
+    tPrime = rng.uniform(0.0, 1.0, size=mass.size)
+    bounds = {"mass": (1.0, 1.8), "tPrime": (0.1, 0.6)}
+    keep = (mass >= 1.0) & (mass < 1.8) & (tPrime >= 0.1) & (tPrime < 0.6)
+    waves = wave_list(nW)
+    names = [w[0] for w in waves]
+    I = PL.ampIntegralMatrix(names)
+    I.setSums(ds["integral"], ds["nMC"])
+    amps = {names[k]: ds["data"][:, k] for k in range(len(names))}
+    half = mass.size // 2
+    files = [{"mass": mass[:half], "tPrime": tPrime[:half]}, {"mass": mass[half:], "tPrime": tPrime[half:]}]
+    Lb = PL.initLikelihood(waves, amps, I, rank=rank, multibinBoundaries=bounds, eventVariables=files)
+    Lf = PL.initLikelihood(waves, {k: v[keep] for k, v in amps.items()}, I, rank=rank)
+    assert Lb.nmbEvents() == Lf.nmbEvents() == int(keep.sum()) > 0
+    sub = dict(ds, data=ds["data"][keep])
+    Lo = H.oracle_likelihood(sub)
+    par = rng.normal(size=Lb.nmbPars())
+    fb, gb = Lb.FdF(par)
+    ff, gf = Lf.FdF(par)
+    assert fb == ff and np.array_equal(gb, gf)
+    fo, go = Lo.fdf(par)
+    assert abs(fb - fo) <= 1e-10 * abs(fo) and H.rel_err(gb, go) <= 1e-10
