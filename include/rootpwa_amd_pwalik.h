@@ -45,6 +45,11 @@ int rpwa_intmat_element(const rpwa_intmat* m, uint32_t i, uint32_t j, double* re
  * same memory space) or NULL */
 int rpwa_intmat_integrate(rpwa_intmat* m, const double* amps, uint64_t nmbEvents, const double* weights,
                           int device, int onDevice);
+/* the same with nmbSubAmps incoherent sub-amplitudes per wave
+ * (ampIntegralMatrix.cc:432-466): amps [nmbSubAmps][M][W] complex,
+ * sum_ij = sum_e weight_e sum_s A_i^s(e) conj(A_j^s(e)), event count M */
+int rpwa_intmat_integrate_incoherent(rpwa_intmat* m, const double* amps, uint32_t nmbSubAmps, uint64_t nmbEvents,
+                                     const double* weights, int device, int onDevice);
 int rpwa_intmat_add(rpwa_intmat* m, const rpwa_intmat* other);   /* operator+= */
 int rpwa_intmat_renormalize(rpwa_intmat* m, uint64_t nmbEvents);
 int rpwa_intmat_write_ascii(const rpwa_intmat* m, const char* fileName);

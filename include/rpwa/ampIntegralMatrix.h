@@ -38,13 +38,14 @@ public:
 	std::complex<double> element(const std::string& waveNameI, const std::string& waveNameJ) const
 	{ return element(waveIndex(waveNameI), waveIndex(waveNameJ)); }
 
-	// ampIntegralMatrix::integrate for one incoherent sub-amplitude per wave:
-	// amps [nmbEvents][nmbWaves] (host or device memory), importance weights w
-	// (the integral uses 1/w) or null.  Replaces the sums and the event count.
+	// ampIntegralMatrix::integrate: amps [nmbSubAmps][nmbEvents][nmbWaves]
+	// complex (host or device memory), the incoherent sub-amplitudes of every
+	// wave (ampIntegralMatrix.cc:432-466), importance weights w per event (the
+	// integral uses 1/w) or null.  Replaces the sums and the event count.
 	bool integrate(const std::complex<double>* amps, const unsigned long nmbEvents,
-	               const double* weights = 0, const int device = 0);
+	               const double* weights = 0, const int device = 0, const unsigned int nmbSubAmps = 1);
 	bool integrateDevice(const double* d_amps, const unsigned long nmbEvents,
-	                     const double* d_weights = 0, const int device = 0);
+	                     const double* d_weights = 0, const int device = 0, const unsigned int nmbSubAmps = 1);
 
 	// ampIntegralMatrix::addEvent (ampIntegralMatrix.cc:280-297), on the host
 	bool addEvent(std::map<std::string, std::complex<double> >& amplitudes);
@@ -69,6 +70,8 @@ public:
 	std::ostream& print(std::ostream& out, const bool printIntegralValues = false) const;
 
 private:
+	bool integrateImpl(const double* amps, const unsigned long nmbEvents, const double* weights, const int device,
+	                   const unsigned int nmbSubAmps, const bool onDevice);
 	std::vector<std::string> _waveNames;
 	std::map<std::string, unsigned int> _waveIndex;
 	unsigned long _nmbEvents;

@@ -74,6 +74,7 @@ def lib():
         L.orc_event_loglik.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
         L.orc_event_deriv.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp]
         L.orc_integral.argtypes = [_dp, ctypes.c_uint64, ctypes.c_uint, _dp, _dp]
+        L.orc_integral_incoherent.argtypes = [_dp, ctypes.c_uint, ctypes.c_uint64, ctypes.c_uint, _dp, _dp]
         L.orc_kahan_float_demo.argtypes = [ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]
         L.orc_gen_hash.restype = ctypes.c_uint64
@@ -259,13 +260,22 @@ class OracleLikelihood:
 
 
 def integral(amps, weights=None):
-    """ampIntegralMatrix::integrate sums, [W][W] complex."""
+    """ampIntegralMatrix::integrate sums, [W][W] complex, of amps [M][W] or,
+    with incoherent sub-amplitudes, [S][M][W]."""
+    a = np.asarray(amps)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    wp = _d(w) if w is not None else None
+    if a.ndim == 3:
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        S, M, W = a.shape
+        out = np.zeros((W, W), dtype=np.complex128)
+        lib().orc_integral_incoherent(a.view(np.float64).ctypes.data_as(_dp), S, M, W, wp,
+                                      out.view(np.float64).ctypes.data_as(_dp))
+        return out
     a = cplx_array(amps)
     M, W = a.shape
     out = np.zeros((W, W), dtype=np.complex128)
-    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-    lib().orc_integral(a.view(np.float64).ctypes.data_as(_dp), M, W, _d(w) if w is not None else None,
-                       out.view(np.float64).ctypes.data_as(_dp))
+    lib().orc_integral(a.view(np.float64).ctypes.data_as(_dp), M, W, wp, out.view(np.float64).ctypes.data_as(_dp))
     return out
 
 

@@ -846,8 +846,11 @@ void orc_lik_correct_param_signs(const orc_lik* L, const double* par, double* ou
 }
 
 /* ------------------------------------------------------------------------- */
-/* ampIntegralMatrix::integrate (ampIntegralMatrix.cc:408-483), one sub-amp  */
-void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weights, double* out)
+/* ampIntegralMatrix::integrate (ampIntegralMatrix.cc:408-483) with S          */
+/* incoherent sub-amplitudes per wave, amps [S][M][W]: the sub-amplitude sum    */
+/* of one event (:462-464) enters the compensated sum of its matrix element     */
+void orc_integral_incoherent(const double* amps, unsigned S, uint64_t M, unsigned W, const double* weights,
+                             double* out)
 {
 	kahan weightAcc = {0, 0};
 	kahanc* ampProdAcc = (kahanc*)calloc((size_t)W * W, sizeof(kahanc));
@@ -858,11 +861,13 @@ void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weig
 			w = weights[e];
 		const double weight = 1 / w;
 		k_add(&weightAcc, weight);
-		const double* A = amps + 2 * (size_t)e * W;
 		for (unsigned i = 0; i < W; ++i)
 			for (unsigned j = 0; j < W; ++j) {
 				cplx val = c_make(0, 0);
-				val = c_add(val, c_mul(c_ld(A + 2 * i), c_conj(c_ld(A + 2 * j))));
+				for (unsigned sub = 0; sub < S; ++sub) {
+					const double* A = amps + 2 * ((size_t)sub * M + e) * W;
+					val = c_add(val, c_mul(c_ld(A + 2 * i), c_conj(c_ld(A + 2 * j))));
+				}
 				if (useWeight)
 					val = c_scale(weight, val);
 				kc_add(&ampProdAcc[(size_t)i * W + j], val);
@@ -879,6 +884,12 @@ void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weig
 			c_st(out + 2 * ((size_t)i * W + j), s);
 		}
 	free(ampProdAcc);
+}
+
+/* one sub-amplitude, amps [M][W] */
+void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weights, double* out)
+{
+	orc_integral_incoherent(amps, 1, M, W, weights, out);
 }
 
 /* testSumAccumulators (test/utilities/testSumAccumulators.cc:46-60) */

@@ -111,6 +111,8 @@ void   orc_event_deriv (const orc_lik* L, const double* prodAmps, double flat,
  * amps: [M][W] complex (event-major); weights: importance weights w_e (the
  * integral uses 1/w_e), or NULL.  out: [W][W] complex sums (the _integrals
  * array, renormalised to the weight integral when weights are given). */
+void orc_integral_incoherent(const double* amps, unsigned S, uint64_t M, unsigned W, const double* weights,
+                             double* out);
 void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weights,
                   double* out);
 

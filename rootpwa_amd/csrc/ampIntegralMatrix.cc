@@ -75,40 +75,52 @@ std::complex<double> ampIntegralMatrix::element(const unsigned int waveIndexI, c
 	return cdiv(_integrals[(size_t)waveIndexI * W + waveIndexJ], (double)_nmbEvents);
 }
 
-bool ampIntegralMatrix::integrate(const std::complex<double>* amps, const unsigned long nmbEvents,
-                                  const double* weights, const int device)
+// integrate (ampIntegralMatrix.cc:300-488): I_ij = sum_e weight_e sum_s A_i^s(e) conj(A_j^s(e)).
+// Each incoherent sub-amplitude s is one rank-k update over the same events and
+// weights (so one mean-weight renormalisation), added on the host in order s = 0, 1, ...
+bool ampIntegralMatrix::integrateImpl(const double* amps, const unsigned long nmbEvents, const double* weights,
+                                      const int device, const unsigned int nmbSubAmps, const bool onDevice)
 {
+	const char* fn = onDevice ? "ampIntegralMatrix::integrateDevice" : "ampIntegralMatrix::integrate";
 	if (_waveNames.empty()) {
-		std::cerr << "ampIntegralMatrix::integrate: set the wave names first" << std::endl;
+		std::cerr << fn << ": set the wave names first" << std::endl;
 		return false;
 	}
-	_integrals.assign(_waveNames.size() * _waveNames.size(), std::complex<double>(0, 0));
-	const int st = rpwa_integral(device, reinterpret_cast<const double*>(amps), nmbEvents, nmbWaves(), weights,
-	                             reinterpret_cast<double*>(_integrals.data()));
-	if (st != RPWA_OK) {
-		std::cerr << "ampIntegralMatrix::integrate: " << rpwa_last_error() << std::endl;
+	if (nmbSubAmps < 1) {
+		std::cerr << fn << ": amplitudes need at least one incoherent sub-amplitude" << std::endl;
 		return false;
 	}
+	const size_t W = nmbWaves();
+	std::vector<std::complex<double> > sums(W * W, std::complex<double>(0, 0));
+	std::vector<std::complex<double> > part(nmbSubAmps > 1 ? W * W : 0);
+	for (unsigned int s = 0; s < nmbSubAmps; ++s) {
+		const double* a = amps + (size_t)s * nmbEvents * W * 2;
+		double* out = reinterpret_cast<double*>(s == 0 ? sums.data() : part.data());
+		const int st = onDevice ? rpwa_integral_device(device, a, nmbEvents, (uint32_t)W, weights, out)
+		                        : rpwa_integral(device, a, nmbEvents, (uint32_t)W, weights, out);
+		if (st != RPWA_OK) {
+			std::cerr << fn << ": " << rpwa_last_error() << std::endl;
+			return false;
+		}
+		if (s > 0)
+			for (size_t k = 0; k < W * W; ++k)
+				sums[k] += part[k];
+	}
+	_integrals.swap(sums);
 	_nmbEvents = nmbEvents;
 	return true;
 }
 
-bool ampIntegralMatrix::integrateDevice(const double* d_amps, const unsigned long nmbEvents,
-                                        const double* d_weights, const int device)
+bool ampIntegralMatrix::integrate(const std::complex<double>* amps, const unsigned long nmbEvents,
+                                  const double* weights, const int device, const unsigned int nmbSubAmps)
 {
-	if (_waveNames.empty()) {
-		std::cerr << "ampIntegralMatrix::integrateDevice: set the wave names first" << std::endl;
-		return false;
-	}
-	_integrals.assign(_waveNames.size() * _waveNames.size(), std::complex<double>(0, 0));
-	const int st = rpwa_integral_device(device, d_amps, nmbEvents, nmbWaves(), d_weights,
-	                                    reinterpret_cast<double*>(_integrals.data()));
-	if (st != RPWA_OK) {
-		std::cerr << "ampIntegralMatrix::integrateDevice: " << rpwa_last_error() << std::endl;
-		return false;
-	}
-	_nmbEvents = nmbEvents;
-	return true;
+	return integrateImpl(reinterpret_cast<const double*>(amps), nmbEvents, weights, device, nmbSubAmps, false);
+}
+
+bool ampIntegralMatrix::integrateDevice(const double* d_amps, const unsigned long nmbEvents,
+                                        const double* d_weights, const int device, const unsigned int nmbSubAmps)
+{
+	return integrateImpl(d_amps, nmbEvents, d_weights, device, nmbSubAmps, true);
 }
 
 // addEvent (ampIntegralMatrix.cc:280-297)

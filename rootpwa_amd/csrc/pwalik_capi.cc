@@ -196,6 +196,26 @@ extern "C" int rpwa_intmat_integrate(rpwa_intmat* m, const double* amps, uint64_
 	GUARD_END
 }
 
+extern "C" int rpwa_intmat_integrate_incoherent(rpwa_intmat* m, const double* amps, uint32_t nmbSubAmps,
+                                                uint64_t nmbEvents, const double* weights, int device, int onDevice)
+{
+	GUARD_BEGIN
+	if (int st = need(m, "rpwa_intmat_integrate_incoherent"))
+		return st;
+	if (nmbSubAmps < 1)
+		return fail("rpwa_intmat_integrate_incoherent", "at least one sub-amplitude is needed");
+	const bool ok = onDevice ? m->m.integrateDevice(amps, nmbEvents, weights, device, nmbSubAmps)
+	                         : m->m.integrate(reinterpret_cast<const std::complex<double>*>(amps), nmbEvents,
+	                                          weights, device, nmbSubAmps);
+	if (not ok) {
+		const std::string prev = rpwa_last_error();
+		set_error("rpwa_intmat_integrate_incoherent failed" + (prev.empty() ? std::string() : ": " + prev));
+		return RPWA_ERR_INVALID;
+	}
+	return RPWA_OK;
+	GUARD_END
+}
+
 extern "C" int rpwa_intmat_add(rpwa_intmat* m, const rpwa_intmat* other)
 {
 	GUARD_BEGIN

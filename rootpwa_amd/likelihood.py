@@ -32,6 +32,7 @@ PROTOTYPES = {
     "rpwa_intmat_set_nmb_events": (_i, [_vp, _u64]),
     "rpwa_intmat_element": (_i, [_vp, _u32, _u32, _dp, _dp]),
     "rpwa_intmat_integrate": (_i, [_vp, _vp, _u64, _vp, _i, _i]),
+    "rpwa_intmat_integrate_incoherent": (_i, [_vp, _vp, _u32, _u64, _vp, _i, _i]),
     "rpwa_intmat_add": (_i, [_vp, _vp]),
     "rpwa_intmat_renormalize": (_i, [_vp, _u64]),
     "rpwa_intmat_write_ascii": (_i, [_vp, ctypes.c_char_p]),
@@ -185,16 +186,25 @@ class ampIntegralMatrix:
         return complex(re.value, im.value)
 
     def integrate(self, amps, weights=None, device=0):
-        """Host amplitudes [M][W] (complex) -> sums on the GPU."""
+        """Host amplitudes [M][W] complex, or [S][M][W] for S incoherent
+        sub-amplitudes per wave -> sums on the GPU."""
         a = np.ascontiguousarray(amps, dtype=np.complex128)
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         self._keep = (a, w)
-        _call("rpwa_intmat_integrate", self.h, a.ctypes.data_as(_vp), a.shape[0],
-              w.ctypes.data_as(_vp) if w is not None else None, int(device), 0)
+        wp = w.ctypes.data_as(_vp) if w is not None else None
+        if a.ndim == 3:
+            if w is not None and w.shape != (a.shape[1],):
+                raise ValueError("one weight per event expected")
+            _call("rpwa_intmat_integrate_incoherent", self.h, a.ctypes.data_as(_vp), a.shape[0], a.shape[1], wp,
+                  int(device), 0)
+        else:
+            if a.ndim != 2 or (w is not None and w.shape != (a.shape[0],)):
+                raise ValueError("amplitudes [M][W] and one weight per event expected")
+            _call("rpwa_intmat_integrate", self.h, a.ctypes.data_as(_vp), a.shape[0], wp, int(device), 0)
 
-    def integrateDevice(self, d_amps, nmbEvents, d_weights=None, device=0):
-        _call("rpwa_intmat_integrate", self.h, _vp(d_amps), int(nmbEvents), _vp(d_weights) if d_weights else None,
-              int(device), 1)
+    def integrateDevice(self, d_amps, nmbEvents, d_weights=None, device=0, nmbSubAmps=1):
+        _call("rpwa_intmat_integrate_incoherent", self.h, _vp(d_amps), int(nmbSubAmps), int(nmbEvents),
+              _vp(d_weights) if d_weights else None, int(device), 1)
 
     def __iadd__(self, other):
         _call("rpwa_intmat_add", self.h, other.h)

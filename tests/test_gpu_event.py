@@ -120,6 +120,30 @@ def test_integral_matches_oracle(gpu):
         assert H.rel_err(gpu.integral(a, w), O.integral(a, w)) <= 1e-12, W
 
 
+def test_integral_incoherent_sub_amplitudes(gpu):
+    """ampIntegralMatrix with two incoherent sub-amplitudes per wave
+    (ampIntegralMatrix.cc:432-466), host and device input."""
+    from rootpwa_amd import likelihood as PL
+    rng = np.random.default_rng(21)
+    S, M, W = 2, 2500, 12
+    a = rng.normal(size=(S, M, W)) + 1j * rng.normal(size=(S, M, W))
+    w = rng.uniform(0.5, 2.0, size=M)
+    names = ["w%02d" % i for i in range(W)]
+    for weights in (None, w):
+        m = PL.ampIntegralMatrix(names)
+        m.integrate(a, weights)
+        sums, n = m.sums()
+        assert n == M and H.rel_err(sums, O.integral(a, weights)) <= 1e-12
+    buf = gpu.DeviceBuffer(a.nbytes)
+    buf.upload(a)
+    wbuf = gpu.DeviceBuffer(w.nbytes)
+    wbuf.upload(w)
+    m = PL.ampIntegralMatrix(names)
+    m.integrateDevice(buf.address, M, wbuf.address, nmbSubAmps=S)
+    sums, n = m.sums()
+    assert n == M and H.rel_err(sums, O.integral(a, w)) <= 1e-12
+
+
 def test_generator_matches_cpu_restatement(gpu):
     seed, W = 77, 12
     scales = O.gen_wave_scales(seed, W)

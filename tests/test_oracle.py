@@ -266,6 +266,25 @@ def test_integral_properties():
     assert H.rel_err(O.integral(a, w), ref) <= 1e-13
 
 
+def test_integral_incoherent_sub_amplitudes():
+    """ampIntegralMatrix.cc:432-466: sub-amplitudes of one event add incoherently."""
+    rng = np.random.default_rng(19)
+    a = rng.normal(size=(2, 400, 5)) + 1j * rng.normal(size=(2, 400, 5))
+    w = rng.uniform(0.5, 2, size=400)
+    # one sub-amplitude is the plain integral
+    assert np.array_equal(O.integral(a[:1]), O.integral(a[0]))
+    assert np.array_equal(O.integral(a[:1], w), O.integral(a[0], w))
+    # a zero second sub-amplitude changes nothing; a duplicated one doubles the sums exactly
+    z = np.stack([a[0], np.zeros_like(a[0])])
+    assert np.array_equal(O.integral(z, w), O.integral(a[0], w))
+    assert np.array_equal(O.integral(np.stack([a[0], a[0]]), w), 2 * O.integral(a[0], w))
+    # sum over events and sub-amplitudes of (1/w) A_i conj(A_j), renormalised by mean(1/w) over events
+    ref = np.einsum("sei,sej->ij", a * (1 / w)[None, :, None], np.conj(a)) / np.mean(1 / w)
+    I = O.integral(a, w)
+    assert H.rel_err(I, ref) <= 1e-13
+    assert H.rel_err(I, O.integral(a[0], w) + O.integral(a[1], w)) <= 1e-13
+
+
 def test_zero_wave_is_fixed():
     # all-zero amplitudes of a wave whose acceptance integral is zero fix its parameters
     # (pwaLikelihood.cc:1248-1282), without changing nmbParsFixed
