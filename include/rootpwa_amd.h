@@ -134,8 +134,9 @@ int rpwa_lik_kernel_times(rpwa_lik_ctx* ctx, double* eventKernel, double* reduce
 int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
 
 /* Event tiles: before the first evaluation of a given rank the context times
- * every feasible kernel variant on its own amplitudes and keeps the fastest
- * for that rank; results of a rank always come from one variant.  Variants:
+ * every feasible kernel variant on its own amplitudes, together with the
+ * reduction of its workgroup rows, and keeps the fastest for that rank;
+ * results of a rank always come from one variant.  Variants:
  * lanes per event in {8,16,32,64} (register tiles: a group of lanes holds one
  * event's amplitude row and the production amplitudes of its columns) and
  * lanes per event 1 (LDS-staged: tiles of 16..64 events pass through LDS, one

@@ -280,8 +280,11 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			a.lanes1 = ts.lanes1;
 			std::vector<float> ms;
 			for (int it = 0; it < reps + 1; ++it) {
+				// the reduction of the workgroup rows is timed with the kernel: its cost
+				// grows with the grid, which differs between the variants
 				RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
 				RPWA_HIP_CHECK(launch_event_kernel(c->lay, ts, a, true, c->nWg[rank], c->stream));
+				RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, 0.1, c->d_out, c->stream));
 				RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
 				RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
 				float t = 0.f;
