@@ -153,6 +153,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched-fit leg")
     ap.add_argument("--skip-hessian", action="store_true", help="skip the analytic-Hessian leg")
     ap.add_argument("--skip-c0", action="store_true", help="skip the small CPU-runnable configuration")
+    ap.add_argument("--skip-fit", action="store_true", help="skip the full-size fits from one start")
     ap.add_argument("--only-batch", action="store_true",
                     help="profiling runs: only the batched leg, printed as its own JSON line")
     ap.add_argument("--initializer", action="store_true",
@@ -273,6 +274,7 @@ def main():
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
     hess = None if args.skip_hessian else hessian_leg(L_64, L_mix, par, N, W, rank, log)
+    fits = None if args.skip_fit else fit_leg(L_64, L_mix, log)
 
     # ---- checks against the oracle at full size ---------------------------------
     check = {}
@@ -293,6 +295,16 @@ def main():
             rel_lnL_fp64=abs(r_64["f"] - f_ref) / abs(f_ref),
             grad_rel_mixed=float(np.max(np.abs(r_mix["g"] - g_ref))) / gscale,
             grad_rel_fp64=float(np.max(np.abs(r_64["g"] - g_ref))) / gscale)
+        if fits:
+            # the FP64 fit's minimum against the oracle: lnL relative, gradient relative to the start's scale
+            pm = fits["fp64"]["par"]
+            fm_ref, gm_ref = Lo.fdf(pm, threads=args.oracle_threads)
+            fm, gm = L_64.FdF(pm)
+            fits["fp64"]["oracle_at_minimum"] = dict(
+                rel_lnL=abs(fm - fm_ref) / abs(fm_ref),
+                grad_err_over_start_scale=float(np.max(np.abs(gm - gm_ref))) / gscale,
+                grad_max_abs_oracle=float(np.max(np.abs(gm_ref))))
+            log("fit minimum vs oracle:", fits["fp64"]["oracle_at_minimum"])
         log("oracle check (%.1f s, %d threads): %s" % (time.perf_counter() - t, args.oracle_threads, check))
         del Lo
 
@@ -382,10 +394,43 @@ def main():
         "c0": c0,
         "batched": batched,
         "hessian": hess,
+        "fits": None if fits is None else {k: {kk: vv for kk, vv in v.items() if kk != "par"} if isinstance(v, dict)
+                                           else v for k, v in fits.items()},
     }
     if cpu:
         out["vs_cpu"] = r_mix["rate"] / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def fit_leg(L_64, L_mix, log):
+    """A Minuit-pattern fit (pwaFit: DoEval, then Gradient) of the FP64 likelihood from one random
+    start at full size, with the Hessian check and covariance; and the NLopt-pattern fit (pwaNloptFit:
+    FdF) of the complex64 likelihood from the start that driver draws for the same seed, to show that
+    it reaches the same minimum."""
+    from rootpwa_amd import likelihood as PL
+    out = {}
+    for key, L, fn in (("fp64", L_64, "pwaFit"), ("mixed", L_mix, "pwaNloptFit")):
+        L.gpuContext().reset_kernel_time()
+        t = time.perf_counter()
+        # each driver draws its random start values from the seed, as the reference's drivers do
+        r = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, checkHessian=True)
+        wall = time.perf_counter() - t
+        ev, red, n = L.gpuContext().kernel_times()
+        # the minimisation alone: no Hessian check, no covariance
+        t = time.perf_counter()
+        r0 = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, saveSpace=True)
+        wall_min = time.perf_counter() - t
+        out[key] = dict(driver=fn, wall_s=wall, minimisation_s=wall_min, lnL=r.logLikelihood(),
+                        converged=r.converged(), edm=r.edm if np.isfinite(r.edm) else None,
+                        evaluations=r.nmbEvaluations, iterations=r.nmbIterations,
+                        kernel_passes=n, kernel_s=ev + red, covariance=r.covMatrixValid(),
+                        non_positive_eigenvalues=r.nmbNonPositiveEigenvalues, par=r.fitParameters,
+                        same_minimum_without_hessian=r0.logLikelihood() == r.logLikelihood())
+        log("%s fit (%s): %.2f s (minimisation alone %.2f s), lnL %.6f, converged %s, %d evaluations, "
+            "%d kernel passes" % (key, fn, wall, wall_min, r.logLikelihood(), r.converged(), r.nmbEvaluations, n))
+    if "fp64" in out and "mixed" in out:
+        out["dlnL_minimum_mixed_vs_fp64"] = abs(out["mixed"]["lnL"] - out["fp64"]["lnL"])
+    return out
 
 
 def hessian_leg(L_64, L_mix, par, N, W, rank, log):

@@ -84,6 +84,158 @@ struct ctxDeleter {
 	}
 };
 
+// Eigen-decomposition of a symmetric n x n matrix: Householder reduction to
+// tridiagonal form with the transformations accumulated, then the implicit QL
+// iteration with Wilkinson-type shifts (the EISPACK routines tred2 and tql2,
+// Wilkinson & Reinsch, Handbook for Automatic Computation II, 1971; the method
+// ROOT's TMatrixDSymEigen implements).  On entry V holds the matrix (row-major);
+// on exit column k of V is the eigenvector of eigenvalue d[k].
+void symmetric_eigen(const int n, std::vector<double>& V, std::vector<double>& d)
+{
+	auto v = [&](int i, int j) -> double& { return V[(size_t)i * n + j]; };
+	d.assign(n, 0.);
+	if (n == 0)
+		return;
+	std::vector<double> e(n, 0.);
+	// ---- tridiagonalisation (rows n-1 .. 1), V accumulates the reflections ----
+	for (int j = 0; j < n; ++j)
+		d[j] = v(n - 1, j);
+	for (int i = n - 1; i > 0; --i) {
+		double scale = 0., h = 0.;
+		for (int k = 0; k < i; ++k)
+			scale += std::fabs(d[k]);
+		if (scale == 0.) {
+			e[i] = d[i - 1];
+			for (int j = 0; j < i; ++j) {
+				d[j] = v(i - 1, j);
+				v(i, j) = 0.;
+				v(j, i) = 0.;
+			}
+		} else {
+			for (int k = 0; k < i; ++k) {
+				d[k] /= scale;
+				h += d[k] * d[k];
+			}
+			double f = d[i - 1];
+			double g = std::sqrt(h);
+			if (f > 0.)
+				g = -g;
+			e[i] = scale * g;
+			h -= f * g;
+			d[i - 1] = f - g;
+			for (int j = 0; j < i; ++j)
+				e[j] = 0.;
+			for (int j = 0; j < i; ++j) {
+				f = d[j];
+				v(j, i) = f;
+				g = e[j] + v(j, j) * f;
+				for (int k = j + 1; k <= i - 1; ++k) {
+					g += v(k, j) * d[k];
+					e[k] += v(k, j) * f;
+				}
+				e[j] = g;
+			}
+			f = 0.;
+			for (int j = 0; j < i; ++j) {
+				e[j] /= h;
+				f += e[j] * d[j];
+			}
+			const double hh = f / (h + h);
+			for (int j = 0; j < i; ++j)
+				e[j] -= hh * d[j];
+			for (int j = 0; j < i; ++j) {
+				f = d[j];
+				g = e[j];
+				for (int k = j; k <= i - 1; ++k)
+					v(k, j) -= (f * e[k] + g * d[k]);
+				d[j] = v(i - 1, j);
+				v(i, j) = 0.;
+			}
+		}
+		d[i] = h;
+	}
+	for (int i = 0; i < n - 1; ++i) {
+		v(n - 1, i) = v(i, i);
+		v(i, i) = 1.;
+		const double h = d[i + 1];
+		if (h != 0.) {
+			for (int k = 0; k <= i; ++k)
+				d[k] = v(k, i + 1) / h;
+			for (int j = 0; j <= i; ++j) {
+				double g = 0.;
+				for (int k = 0; k <= i; ++k)
+					g += v(k, i + 1) * v(k, j);
+				for (int k = 0; k <= i; ++k)
+					v(k, j) -= g * d[k];
+			}
+		}
+		for (int k = 0; k <= i; ++k)
+			v(k, i + 1) = 0.;
+	}
+	for (int j = 0; j < n; ++j) {
+		d[j] = v(n - 1, j);
+		v(n - 1, j) = 0.;
+	}
+	v(n - 1, n - 1) = 1.;
+	e[0] = 0.;
+	// ---- implicit QL on the tridiagonal matrix (diagonal d, sub-diagonal e) ----
+	for (int i = 1; i < n; ++i)
+		e[i - 1] = e[i];
+	e[n - 1] = 0.;
+	double f = 0., tst1 = 0.;
+	const double eps = std::ldexp(1., -52);
+	for (int l = 0; l < n; ++l) {
+		tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+		int m = l;
+		while (m < n - 1 && std::fabs(e[m]) > eps * tst1)
+			++m;
+		if (m > l) {
+			for (int iter = 0; iter < 60; ++iter) {
+				double g = d[l];
+				double p = (d[l + 1] - g) / (2. * e[l]);
+				double r = std::hypot(p, 1.);
+				if (p < 0.)
+					r = -r;
+				d[l] = e[l] / (p + r);
+				d[l + 1] = e[l] * (p + r);
+				const double dl1 = d[l + 1];
+				double h = g - d[l];
+				for (int i = l + 2; i < n; ++i)
+					d[i] -= h;
+				f += h;
+				p = d[m];
+				double c = 1., c2 = 1., c3 = 1., s = 0., s2 = 0.;
+				const double el1 = e[l + 1];
+				for (int i = m - 1; i >= l; --i) {
+					c3 = c2;
+					c2 = c;
+					s2 = s;
+					g = c * e[i];
+					h = c * p;
+					r = std::hypot(p, e[i]);
+					e[i + 1] = s * r;
+					s = e[i] / r;
+					c = p / r;
+					p = c * d[i] - s * g;
+					d[i + 1] = h + s * (c * g + s * d[i]);
+					for (int k = 0; k < n; ++k) {
+						h = v(k, i + 1);
+						v(k, i + 1) = s * v(k, i) + c * h;
+						v(k, i) = c * v(k, i) - s * h;
+					}
+				}
+				p = -s * s2 * c3 * el1 * e[l] / dl1;
+				e[l] = s * p;
+				d[l] = c * p;
+				if (!(std::fabs(e[l]) > eps * tst1))
+					break;
+			}
+		}
+		d[l] += f;
+		e[l] = 0.;
+	}
+}
+
 template <typename complexT>
 int gpu_precision() { return RPWA_PREC_FP64; }
 template <>
@@ -990,11 +1142,9 @@ denseMatrix pwaLikelihood<complexT>::CovarianceMatrix(const double* par) const
 }
 
 // HessianEigenVectors (pwaLikelihood.cc:864-911): eigen-decomposition of the
-// Hessian reduced to the free parameters; eigenvectors are expanded back to
-// all parameters with zeros at the fixed ones.  Order: descending eigenvalues.
-// The decomposition is the cyclic Jacobi method (Golub & Van Loan, Matrix
-// Computations, sec. 8.5): exact rotations until the off-diagonal norm is
-// below 1e-15 of the Frobenius norm.
+// Hessian reduced to the free parameters (symmetric_eigen above); eigenvectors
+// are expanded back to all parameters with zeros at the fixed ones.  Order:
+// descending eigenvalues.
 template <typename complexT>
 std::vector<std::pair<std::vector<double>, double> >
 pwaLikelihood<complexT>::HessianEigenVectors(const denseMatrix& hessian) const
@@ -1004,59 +1154,17 @@ pwaLikelihood<complexT>::HessianEigenVectors(const denseMatrix& hessian) const
 		if (not _parameters[i].fixed())
 			freeIdx.push_back(i);
 	const unsigned int n = (unsigned int)freeIdx.size();
-	std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.);
-	double frob = 0.;
-	for (unsigned int i = 0; i < n; ++i) {
-		v[(size_t)i * n + i] = 1.;
-		for (unsigned int j = 0; j < n; ++j) {
+	std::vector<double> v((size_t)n * n), lambda;
+	for (unsigned int i = 0; i < n; ++i)
+		for (unsigned int j = 0; j < n; ++j)
 			// symmetrise: the reference's Hessian is symmetric by construction
-			const double h = 0.5 * (hessian(freeIdx[i], freeIdx[j]) + hessian(freeIdx[j], freeIdx[i]));
-			a[(size_t)i * n + j] = h;
-			frob += h * h;
-		}
-	}
-	const double target = 1e-30 * frob;
-	for (unsigned int sweep = 0; sweep < 100; ++sweep) {
-		double off = 0.;
-		for (unsigned int p = 0; p < n; ++p)
-			for (unsigned int q = p + 1; q < n; ++q)
-				off += 2. * a[(size_t)p * n + q] * a[(size_t)p * n + q];
-		if (off <= target)
-			break;
-		for (unsigned int p = 0; p + 1 < n; ++p)
-			for (unsigned int q = p + 1; q < n; ++q) {
-				const double apq = a[(size_t)p * n + q];
-				if (apq == 0.)
-					continue;
-				// rotation that zeroes a[p][q]: t = tan(theta), the smaller root of
-				// t^2 + 2 tau t - 1 = 0 with tau = (a_qq - a_pp) / (2 a_pq)
-				const double tau = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2. * apq);
-				const double t = (tau >= 0. ? 1. : -1.) / (std::fabs(tau) + std::sqrt(1. + tau * tau));
-				const double c = 1. / std::sqrt(1. + t * t);
-				const double s = t * c;
-				for (unsigned int k = 0; k < n; ++k) {  // columns p, q
-					const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-					a[(size_t)k * n + p] = c * akp - s * akq;
-					a[(size_t)k * n + q] = s * akp + c * akq;
-				}
-				for (unsigned int k = 0; k < n; ++k) {  // rows p, q
-					const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-					a[(size_t)p * n + k] = c * apk - s * aqk;
-					a[(size_t)q * n + k] = s * apk + c * aqk;
-				}
-				a[(size_t)p * n + q] = a[(size_t)q * n + p] = 0.;
-				for (unsigned int k = 0; k < n; ++k) {
-					const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-					v[(size_t)k * n + p] = c * vkp - s * vkq;
-					v[(size_t)k * n + q] = s * vkp + c * vkq;
-				}
-			}
-	}
+			v[(size_t)i * n + j] = 0.5 * (hessian(freeIdx[i], freeIdx[j]) + hessian(freeIdx[j], freeIdx[i]));
+	symmetric_eigen((int)n, v, lambda);
 	std::vector<unsigned int> order(n);
 	for (unsigned int i = 0; i < n; ++i)
 		order[i] = i;
 	std::stable_sort(order.begin(), order.end(), [&](unsigned int x, unsigned int y) {
-		return a[(size_t)x * n + x] > a[(size_t)y * n + y];
+		return lambda[x] > lambda[y];
 	});
 	std::vector<std::pair<std::vector<double>, double> > eigen;
 	eigen.reserve(n);
@@ -1064,7 +1172,7 @@ pwaLikelihood<complexT>::HessianEigenVectors(const denseMatrix& hessian) const
 		std::vector<double> vec(_nmbPars, 0.);
 		for (unsigned int j = 0; j < n; ++j)
 			vec[freeIdx[j]] = v[(size_t)j * n + order[k]];
-		eigen.push_back(std::make_pair(vec, a[(size_t)order[k] * n + order[k]]));
+		eigen.push_back(std::make_pair(vec, lambda[order[k]]));
 	}
 	return eigen;
 }

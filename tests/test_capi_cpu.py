@@ -215,6 +215,36 @@ def test_hessian_eigenvectors_against_lapack():
         assert abs(abs(v[free] @ ref_vecs[:, k]) - 1.0) < 1e-8
 
 
+@pytest.mark.parametrize("kind", ["degenerate", "indefinite", "large"])
+def test_hessian_eigenvectors_special_spectra(kind):
+    """Repeated eigenvalues (any orthonormal basis of the eigenspace is right),
+    negative eigenvalues (a Hessian away from the minimum), and the size of the
+    2M x 88 rank-2 fit (345 parameters): A v = lambda v, V orthonormal,
+    eigenvalues equal to LAPACK's."""
+    from rootpwa_amd import likelihood as PL
+    nW = (8, 80) if kind == "large" else (2, 5)
+    L = PL.pwaLikelihood()
+    L.init(wave_list(nW), 2)
+    P = L.nmbPars()
+    rng = np.random.default_rng(8)
+    Q, _ = np.linalg.qr(rng.normal(size=(P, P)))
+    if kind == "degenerate":
+        lam = np.repeat([5.0, 2.0, 2.0, -1.0], -(-P // 4))[:P]
+    elif kind == "indefinite":
+        lam = rng.normal(size=P) * 10
+    else:
+        lam = np.exp(rng.normal(size=P) * 3)
+    Hm = (Q * lam) @ Q.T
+    eig = L.HessianEigenVectors(Hm)
+    vals = np.array([e[1] for e in eig])
+    V = np.array([e[0] for e in eig])
+    scale = np.max(np.abs(lam))
+    assert np.all(np.diff(vals) <= 0)
+    assert np.max(np.abs(vals - np.sort(lam)[::-1])) <= 1e-12 * scale
+    assert np.max(np.abs(V @ V.T - np.eye(P))) <= 1e-12
+    assert max(np.linalg.norm(Hm @ V[k] - vals[k] * V[k]) for k in range(P)) <= 1e-12 * scale
+
+
 def test_fit_drivers_refuse_unfinished_likelihood():
     from rootpwa_amd import capi
     from rootpwa_amd import likelihood as PL
