@@ -44,10 +44,14 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double x)
 }
 
 // Second stage: fixed-order sum of the workgroup partial rows.
-// Block = 16 elements x 64 chunks of workgroups (a few rows per thread, so the
-// sum is not bound by one thread's load latency); chunks are combined in order.
+// Block = 16 elements x 64 chunks of workgroups: each thread adds the rows of
+// its chunk (loads issued kRedBatch at a time), then the 64 chunk sums of an
+// element are combined by a fixed pairwise tree in LDS.  Element 0 (the lnL
+// sum, with element 1 its compensation) is summed with Neumaier steps in the
+// chunks and combined with error-free additions (TwoSum) in the tree.
 constexpr int kRedElems = 16;
 constexpr int kRedChunks = 64;
+constexpr int kRedBatch = 8;
 
 __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
     const double* __restrict__ partials, int nWg, uint32_t nElem, double flat,
@@ -61,51 +65,67 @@ __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
 	const int w0 = (int)((int64_t)nWg * ch / kRedChunks);
 	const int w1 = (int)((int64_t)nWg * (ch + 1) / kRedChunks);
 	double s = 0., c = 0.;
-	if (elem < nElem) {
-		if (elem < 2) {
-			// lnL: compensated over both the sums and their corrections
-			for (int w = w0; w < w1; ++w) {
-				neumaier_add(s, c, partials[(size_t)w * nElem + 0]);
-				neumaier_add(s, c, partials[(size_t)w * nElem + 1]);
+	if (elem == 0) {
+		// lnL: compensated over both the sums and their corrections (the loads of a
+		// batch are issued before the dependent additions)
+		const double* q = partials + (size_t)w0 * nElem;
+		int w = w0;
+		for (; w + kRedBatch <= w1; w += kRedBatch, q += (size_t)kRedBatch * nElem) {
+			double2 v[kRedBatch];
+#pragma unroll
+			for (int k = 0; k < kRedBatch; ++k)
+				v[k] = make_double2(q[(size_t)k * nElem], q[(size_t)k * nElem + 1]);
+#pragma unroll
+			for (int k = 0; k < kRedBatch; ++k) {
+				neumaier_add(s, c, v[k].x);
+				neumaier_add(s, c, v[k].y);
 			}
-		} else {
-			int w = w0;
-			for (; w + 4 <= w1; w += 4) {
-				const double v0 = partials[(size_t)(w + 0) * nElem + elem];
-				const double v1 = partials[(size_t)(w + 1) * nElem + elem];
-				const double v2 = partials[(size_t)(w + 2) * nElem + elem];
-				const double v3 = partials[(size_t)(w + 3) * nElem + elem];
-				s += v0;
-				s += v1;
-				s += v2;
-				s += v3;
-			}
-			for (; w < w1; ++w)
-				s += partials[(size_t)w * nElem + elem];
 		}
+		for (; w < w1; ++w, q += nElem) {
+			neumaier_add(s, c, q[0]);
+			neumaier_add(s, c, q[1]);
+		}
+	} else if (elem > 1 && elem < nElem) {
+		const double* q = partials + (size_t)w0 * nElem + elem;
+		int w = w0;
+		for (; w + kRedBatch <= w1; w += kRedBatch, q += (size_t)kRedBatch * nElem) {
+			double v[kRedBatch];
+#pragma unroll
+			for (int k = 0; k < kRedBatch; ++k)
+				v[k] = q[(size_t)k * nElem];
+#pragma unroll
+			for (int k = 0; k < kRedBatch; ++k)
+				s += v[k];
+		}
+		for (; w < w1; ++w, q += nElem)
+			s += q[0];
 	}
 	sh[ch][el] = s;
 	shc[ch][el] = c;
 	__syncthreads();
-	if (ch == 0 && elem < nElem) {
-		if (elem < 2) {
-			double ts = 0., tc = 0.;
-			for (int k = 0; k < kRedChunks; ++k) {
-				neumaier_add(ts, tc, sh[k][el]);
-				neumaier_add(ts, tc, shc[k][el]);
+	// chunk k absorbs chunk k + step: ((0+32)+(16+48))+... a fixed tree
+	for (int step = kRedChunks / 2; step > 0; step >>= 1) {
+		if (ch < step) {
+			const double a = sh[ch][el], b = sh[ch + step][el];
+			const double t = a + b;
+			if (elem == 0) {
+				const double bp = t - a;
+				const double err = (a - (t - bp)) + (b - bp);  // TwoSum: a + b = t + err exactly
+				shc[ch][el] = (shc[ch][el] + shc[ch + step][el]) + err;
 			}
-			if (elem == 0)
-				out[0] = ts + tc;
-		} else {
-			double t = 0.;
-			for (int k = 0; k < kRedChunks; ++k)
-				t += sh[k][el];
-			// out[1] = flat * sum_e(-2/sigma_e); out[2..] = derivatives
-			if (elem == 2)
-				out[1] = flat * t;
-			else
-				out[elem - 1] = t;
+			sh[ch][el] = t;
 		}
+		__syncthreads();
+	}
+	if (ch == 0 && elem < nElem && elem != 1) {
+		const double t = sh[0][el];
+		// out[0] = lnL, out[1] = flat * sum_e(-2/sigma_e), out[2..] = derivatives
+		if (elem == 0)
+			out[0] = t + shc[0][el];
+		else if (elem == 2)
+			out[1] = flat * t;
+		else
+			out[elem - 1] = t;
 	}
 }
 
