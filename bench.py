@@ -405,22 +405,24 @@ def main():
 def fit_leg(L_64, L_mix, log):
     """A Minuit-pattern fit (pwaFit: DoEval, then Gradient) of the FP64 likelihood from one random
     start at full size, with the Hessian check and covariance; and the NLopt-pattern fit (pwaNloptFit:
-    FdF) of the complex64 likelihood from the start that driver draws for the same seed, to show that
-    it reaches the same minimum."""
+    FdF) of both the FP64 and the complex64 likelihood from the start that driver draws for the same
+    seed, to show that complex64 storage reaches the same minimum."""
     from rootpwa_amd import likelihood as PL
     out = {}
-    for key, L, fn in (("fp64", L_64, "pwaFit"), ("mixed", L_mix, "pwaNloptFit")):
+    for key, L, fn in (("fp64", L_64, "pwaFit"), ("fp64_nlopt", L_64, "pwaNloptFit"), ("mixed", L_mix, "pwaNloptFit")):
         L.gpuContext().reset_kernel_time()
         t = time.perf_counter()
-        # each driver draws its random start values from the seed, as the reference's drivers do
-        r = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, checkHessian=True)
+        # each driver draws its random start values from the seed, as the reference's drivers do; the
+        # NLopt pair runs to tight tolerances so that its minima compare the precisions, not the stopping
+        tight = dict(xtolRel=1e-10, ftolAbs=1e-9) if fn == "pwaNloptFit" else {}
+        r = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, checkHessian=True, **tight)
         wall = time.perf_counter() - t
         ev, red, n = L.gpuContext().kernel_times()
         # the minimisation alone: no Hessian check, no covariance
         t = time.perf_counter()
-        r0 = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, saveSpace=True)
+        r0 = (PL.pwaFit if fn == "pwaFit" else PL.pwaNloptFit)(L, seed=SEED, saveSpace=True, **tight)
         wall_min = time.perf_counter() - t
-        out[key] = dict(driver=fn, wall_s=wall, minimisation_s=wall_min, lnL=r.logLikelihood(),
+        out[key] = dict(driver=fn, options=tight or "driver defaults", wall_s=wall, minimisation_s=wall_min, lnL=r.logLikelihood(),
                         converged=r.converged(), edm=r.edm if np.isfinite(r.edm) else None,
                         evaluations=r.nmbEvaluations, iterations=r.nmbIterations,
                         kernel_passes=n, kernel_s=ev + red, covariance=r.covMatrixValid(),
@@ -428,8 +430,15 @@ def fit_leg(L_64, L_mix, log):
                         same_minimum_without_hessian=r0.logLikelihood() == r.logLikelihood())
         log("%s fit (%s): %.2f s (minimisation alone %.2f s), lnL %.6f, converged %s, %d evaluations, "
             "%d kernel passes" % (key, fn, wall, wall_min, r.logLikelihood(), r.converged(), r.nmbEvaluations, n))
-    if "fp64" in out and "mixed" in out:
-        out["dlnL_minimum_mixed_vs_fp64"] = abs(out["mixed"]["lnL"] - out["fp64"]["lnL"])
+    # the same driver and start on both precisions: the difference of the minima is the precision's alone
+    # (pwaFit and pwaNloptFit stop at different tolerances from different starts)
+    f_at = L_64.DoEval(out["mixed"]["par"])
+    out["mixed_vs_fp64_same_driver"] = dict(
+        dlnL_minima=abs(out["mixed"]["lnL"] - out["fp64_nlopt"]["lnL"]),
+        fp64_lnL_at_mixed_minimum_minus_fp64_minimum=f_at - out["fp64_nlopt"]["lnL"],
+        mixed_lnL_error_at_its_minimum=abs(out["mixed"]["lnL"] - f_at),
+        max_parameter_difference=float(np.max(np.abs(out["mixed"]["par"] - out["fp64_nlopt"]["par"]))))
+    log("complex64 vs FP64 minimum (pwaNloptFit, same start):", out["mixed_vs_fp64_same_driver"])
     return out
 
 
