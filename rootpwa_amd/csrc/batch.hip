@@ -329,8 +329,15 @@ struct BatchReduceArgs {
 	uint32_t W, Wp, P, rowLen;
 };
 
-// per fit: gradient partials in chunk order, norm term, value and gradient
-__global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs r)
+// per fit: gradient partials in chunk order, norm term, value and gradient.
+// 256 threads: thread (h, i) adds half h of the bin's chunks for wave component
+// i (the halves are then added in order 0, 1), and the norm term's row sums
+// read T from LDS with the loads of eight columns issued together.
+constexpr int kRedThreads = 256;
+constexpr int kRedHalf = kRedThreads / 2;
+static_assert(RPWA_BATCH_MAX_WAVES <= kRedHalf, "one thread per wave component and half");
+
+__global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchReduceArgs r)
 {
 	const uint32_t b = blockIdx.x / kCols;
 	const int col = r.startCol[blockIdx.x];
@@ -338,19 +345,25 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 		return;
 	const uint32_t q = (uint32_t)col / kGroup, jc = (uint32_t)col % kGroup;
 	const uint32_t c0 = r.binChunk0[b], c1 = r.binChunk0[b + 1];
+	const uint32_t cm = c0 + (c1 - c0 + 1) / 2;
 	const size_t fit = blockIdx.x;
-	const uint32_t i = threadIdx.x;
-	__shared__ double red[3][128];
+	const uint32_t tid = threadIdx.x;
+	const uint32_t i = tid % kRedHalf, h = tid / kRedHalf;
+	__shared__ double red[3][kRedThreads];
+	__shared__ double2 gh[kRedHalf];   // the second half's gradient sums
+	__shared__ double2 Ts[kRedHalf];   // T of the fit
 	const float* Tf = r.T32 + fit * r.rowLen;
-	double normPart = 0.;
+	if (tid < r.W)
+		Ts[tid] = make_double2((double)Tf[2 * tid], (double)Tf[2 * tid + 1]);
+	double gre = 0., gim = 0.;
 	if (i < r.W) {
 		// chunk partials of wave component i, added in chunk order; the loads of
 		// eight chunks are issued before their adds
-		double gre = 0., gim = 0.;
 		const size_t cstride = (size_t)4 * r.Wp * 2 * kGroup;  // floats between consecutive chunks
-		const float* pg = r.partG + ((size_t)(c0 * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
-		uint32_t c = c0;
-		for (; c + 8 <= c1; c += 8, pg += 8 * cstride) {
+		const uint32_t ca = h ? cm : c0, cb = h ? c1 : cm;
+		const float* pg = r.partG + ((size_t)(ca * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
+		uint32_t c = ca;
+		for (; c + 8 <= cb; c += 8, pg += 8 * cstride) {
 			float2 v[8];
 #pragma unroll
 			for (int k = 0; k < 8; ++k)
@@ -361,21 +374,47 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 				gim += (double)v[k].y;
 			}
 		}
-		for (; c < c1; ++c, pg += cstride) {
+		for (; c < cb; ++c, pg += cstride) {
 			gre += (double)pg[0];
 			gim += (double)pg[1];
 		}
+		if (h)
+			gh[i] = make_double2(gre, gim);
+	}
+	// the per-chunk log sums and weight sums: thread t takes chunks c0+t, c0+t+256, ...
+	double lnPart = 0., invPart = 0.;
+	for (uint32_t c = c0 + tid; c < c1; c += kRedThreads) {
+		lnPart += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
+		invPart += r.partInv[(size_t)(c * 4 + q) * kGroup + jc];
+	}
+	__syncthreads();
+	double normPart = 0.;
+	if (h == 0 && i < r.W) {
+		gre += gh[i].x;
+		gim += gh[i].y;
 		// u_i = sum_j T_j conj(I_ij); norm = sum_i Re(T_i conj(u_i)); d norm / d T_i = 2 u_i
-		const double* I = r.acc + ((size_t)b * r.W + i) * r.W * 2;
+		const double2* I = reinterpret_cast<const double2*>(r.acc) + ((size_t)b * r.W + i) * r.W;
 		double ure = 0., uim = 0.;
-		for (uint32_t jw = 0; jw < r.W; ++jw) {
-			const double tr = Tf[2 * jw], ti = Tf[2 * jw + 1];
-			const double ir = I[2 * jw], ii = I[2 * jw + 1];
-			ure += tr * ir + ti * ii;
-			uim += ti * ir - tr * ii;
+		uint32_t jw = 0;
+		for (; jw + 8 <= r.W; jw += 8) {
+			double2 iv[8];
+#pragma unroll
+			for (int k = 0; k < 8; ++k)
+				iv[k] = I[jw + k];
+#pragma unroll
+			for (int k = 0; k < 8; ++k) {
+				const double2 t = Ts[jw + k];
+				ure += t.x * iv[k].x + t.y * iv[k].y;
+				uim += t.y * iv[k].x - t.x * iv[k].y;
+			}
 		}
-		const double tir = Tf[2 * i], tii = Tf[2 * i + 1];
-		normPart = tir * ure + tii * uim;
+		for (; jw < r.W; ++jw) {
+			const double2 t = Ts[jw], iv = I[jw];
+			ure += t.x * iv.x + t.y * iv.y;
+			uim += t.y * iv.x - t.x * iv.y;
+		}
+		const double2 ti = Ts[i];
+		normPart = ti.x * ure + ti.y * uim;
 		const double dre = -2. * gre + 2. * ure, dim = -2. * gim + 2. * uim;
 		double* gout = r.gt + fit * r.P;
 		if (i == 0)
@@ -385,25 +424,19 @@ __global__ __launch_bounds__(128) void batch_reduce_kernel(const BatchReduceArgs
 			gout[2 * i] = dim;
 		}
 	}
-	// the per-chunk log sums and weight sums: thread i takes chunks c0+i, c0+i+128, ...
-	double lnPart = 0., invPart = 0.;
-	for (uint32_t c = c0 + i; c < c1; c += 128) {
-		lnPart += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
-		invPart += r.partInv[(size_t)(c * 4 + q) * kGroup + jc];
-	}
-	red[0][i] = normPart;
-	red[1][i] = lnPart;
-	red[2][i] = invPart;
+	red[0][tid] = normPart;
+	red[1][tid] = lnPart;
+	red[2][tid] = invPart;
 	__syncthreads();
-	for (uint32_t w = 64; w > 0; w >>= 1) {  // fixed-order tree over the 128 threads
-		if (i < w) {
-			red[0][i] += red[0][i + w];
-			red[1][i] += red[1][i + w];
-			red[2][i] += red[2][i + w];
+	for (uint32_t w = kRedThreads / 2; w > 0; w >>= 1) {  // fixed-order tree over the threads
+		if (tid < w) {
+			red[0][tid] += red[0][tid + w];
+			red[1][tid] += red[1][tid + w];
+			red[2][tid] += red[2][tid + w];
 		}
 		__syncthreads();
 	}
-	if (i == 0) {
+	if (tid == 0) {
 		const double lnSum = red[1][0], invSum = red[2][0];
 		const double flat = r.xt[fit * r.P + r.P - 1];
 		const double eta = r.totAcc[b];
@@ -926,7 +959,8 @@ int launch_event(rpwa_batch* b)
 
 int launch_reduce(rpwa_batch* b)
 {
-	hipLaunchKernelGGL(batch_reduce_kernel, dim3((uint32_t)nFits(b)), dim3(128), 0, b->stream, reduce_args(b));
+	hipLaunchKernelGGL(batch_reduce_kernel, dim3((uint32_t)nFits(b)), dim3(kRedThreads), 0, b->stream,
+	                   reduce_args(b));
 	RPWA_HIP_CHECK(hipGetLastError());
 	return RPWA_OK;
 }
