@@ -384,3 +384,19 @@ def test_on_the_fly_binning_selection():
     assert L.nmbEvents() == 3
     with pytest.raises(capi.RpwaError):       # binning after the amplitudes
         L.setOnTheFlyBinning(bounds, {"mass": m, "tPrime": t})
+
+
+def test_cpp_consumer_links_and_sets_up():
+    """tests/cpp/pwalik_dropin.cc, a C++ program using rpwa::pwaLikelihood as
+    ROOTPWA's drivers do, links against the library (built by
+    __graft_entry__.build()); without a GPU it sets up and names the
+    parameters, and an evaluation before finishInit throws."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "pwalik_dropin")
+    assert os.path.exists(exe), "build() links tests/cpp/pwalik_dropin"
+    r = subprocess.run([exe, "--host-only"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].split() == ["V0_neg00_RE", "V0_pos00_RE", "V0_pos01_RE", "V0_pos01_IM", "V1_pos01_RE",
+                                "V_flat_RE"]
+    assert lines[1] == "evaluation before finishInit throws"

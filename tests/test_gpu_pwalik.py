@@ -188,3 +188,38 @@ def test_on_the_fly_binning_matches_prefiltered(gpu):
     assert fb == ff and np.array_equal(gb, gf)
     fo, go = Lo.fdf(par)
     assert abs(fb - fo) <= 1e-10 * abs(fo) and H.rel_err(gb, go) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_cpp_consumer_matches_oracle(gpu, tmp_path, precision):
+    """The C++ consumer (tests/cpp/pwalik_dropin.cc: rpwa::pwaLikelihood through
+    IGradientFunctionMultiDim and Clone, as pwaFit / pwaNloptFit call it) against
+    the oracle, with its bitwise DoEval / Gradient / DoDerivative contracts."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "pwalik_dropin")
+    assert os.path.exists(exe), "build() links tests/cpp/pwalik_dropin"
+    nW, rank = (3, 7), 2
+    ds = H.dataset(nW, rank, 600, 2000, 91)
+    Lo = H.oracle_likelihood(ds)
+    rng = np.random.default_rng(92)
+    pts = [rng.normal(size=Lo.nPars) * s for s in (0.5, 1.0, 2.0)]
+    W = sum(nW)
+    with open(tmp_path / "in.txt", "w") as fh:
+        fh.write("%d %d %d %d %d %d\n" % (nW[0], nW[1], rank, ds["data"].shape[0], ds["nMC"], precision))
+        fh.write(" ".join("%.17g %.17g" % (z.real, z.imag) for z in ds["integral"].ravel()) + "\n")
+        fh.write(" ".join("%.17g %.17g" % (z.real, z.imag) for z in ds["data"].ravel()) + "\n")
+        fh.write("%d\n" % len(pts))
+        for p in pts:
+            fh.write(" ".join("%.17g" % x for x in p) + "\n")
+    r = subprocess.run([exe, str(tmp_path / "in.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert len(lines) == 2 * len(pts)
+    tol_f, tol_g = (1e-12, 1e-10) if precision == 0 else (1e-6, 1e-5)
+    for k, p in enumerate(pts):
+        vals = np.array([float(x) for x in lines[2 * k].split()])
+        assert lines[2 * k + 1] == "bitwise 1"
+        f_ref, g_ref = Lo.fdf(p)
+        assert abs(vals[0] - f_ref) <= tol_f * abs(f_ref)
+        assert H.rel_err(vals[1:], g_ref) <= tol_g
