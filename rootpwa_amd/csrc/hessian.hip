@@ -28,59 +28,77 @@ __device__ __forceinline__ double2 amp_at(const void* amps, uint64_t e, uint32_t
 	return reinterpret_cast<const double2*>(amps)[e * rowLen + pos];
 }
 
+// One block handles nEv consecutive events: their amplitude rows are staged
+// in LDS with coalesced loads (compact wave order: the refl -1 waves, then the
+// refl +1 waves), then S, the weights and the rows are computed with the
+// threads spread over (event, entry), so that every global store is coalesced.
 template <bool MIXED>
-__global__ void hessian_rows_kernel(const void* amps, uint64_t e0, uint64_t n, uint32_t rowLen, uint32_t secOff1,
-                                    uint32_t nW0, uint32_t nW1, uint32_t wmax, int rank,
-                                    const double2* __restrict__ prod, double flat2, uint32_t nd,
-                                    double2* __restrict__ dRows, double2* __restrict__ aRows,
-                                    double* __restrict__ wD, double* __restrict__ wA)
+__global__ __launch_bounds__(256) void hessian_rows_kernel(const void* amps, uint64_t e0, uint64_t n,
+                                                           uint32_t nEv, uint32_t rowLen, uint32_t secOff1,
+                                                           uint32_t nW0, uint32_t nW1, uint32_t wmax, int rank,
+                                                           const double2* __restrict__ prod, double flat2,
+                                                           uint32_t nd, double2* __restrict__ dRows,
+                                                           double2* __restrict__ aRows, double* __restrict__ wD,
+                                                           double* __restrict__ wA)
 {
-	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n)
-		return;
-	const uint64_t e = e0 + k;
+	extern __shared__ double2 lds[];
 	const uint32_t W = nW0 + nW1;
-	double2 S[kMaxRank][2];
-	double sigma = 0.;
-	for (int r = 0; r < kMaxRank; ++r)
-		for (int s = 0; s < 2; ++s) {
-			double2 acc = make_double2(0., 0.);
-			const uint32_t nw = s ? nW1 : nW0;
-			if (r < rank)
-				for (uint32_t i = 0; i < nw; ++i) {
-					const double2 t = prod[((size_t)r * 2 + s) * wmax + i];
-					const double2 x = amp_at<MIXED>(amps, e, rowLen, (s ? secOff1 : 0) + i);
-					acc.x = fma(t.x, x.x, acc.x);
-					acc.x = fma(-t.y, x.y, acc.x);
-					acc.y = fma(t.x, x.y, acc.y);
-					acc.y = fma(t.y, x.x, acc.y);
-				}
-			S[r][s] = acc;
-			sigma += acc.x * acc.x + acc.y * acc.y;
-		}
-	sigma += flat2;
-	const double factor = 2. / sigma;
-	wD[k] = factor * factor;
-	wA[k] = -factor;
-	double2* drow = dRows + k * (nd + 1);
-	uint32_t q = 0;
-	for (int r = 0; r < kMaxRank; ++r) {
-		if (r >= rank)
-			break;
-		for (int s = 0; s < 2; ++s) {
-			const uint32_t nw = s ? nW1 : nW0;
-			for (uint32_t i = 0; i < nw; ++i) {
-				const double2 x = amp_at<MIXED>(amps, e, rowLen, (s ? secOff1 : 0) + i);
-				// d = S conj(A)
-				drow[q++] = make_double2(S[r][s].x * x.x + S[r][s].y * x.y, S[r][s].y * x.x - S[r][s].x * x.y);
-			}
-		}
+	const uint32_t R2 = 2 * (uint32_t)rank;
+	double2* A = lds;                           // [nEv][W]
+	double2* Sx = lds + (size_t)nEv * W;        // [nEv][rank][2]
+	const uint64_t k0 = (uint64_t)blockIdx.x * nEv;
+	const uint32_t ne = (uint32_t)min((uint64_t)nEv, n - k0);
+	for (uint32_t idx = threadIdx.x; idx < ne * W; idx += blockDim.x) {
+		const uint32_t ev = idx / W, g = idx - ev * W;
+		A[idx] = amp_at<MIXED>(amps, e0 + k0 + ev, rowLen, g < nW0 ? g : secOff1 + (g - nW0));
 	}
-	drow[nd] = make_double2(1., 0.);
-	double2* arow = aRows + k * (W + 1);
-	for (uint32_t i = 0; i < W; ++i)
-		arow[i] = amp_at<MIXED>(amps, e, rowLen, i < nW0 ? i : secOff1 + (i - nW0));
-	arow[W] = make_double2(1., 0.);
+	__syncthreads();
+	// S_rs(e) = sum_i T_rsi A_si(e), in the order of the FdF kernels' host restatement
+	for (uint32_t idx = threadIdx.x; idx < ne * R2; idx += blockDim.x) {
+		const uint32_t ev = idx / R2, rs = idx - ev * R2, r = rs / 2, s = rs % 2;
+		const uint32_t nw = s ? nW1 : nW0, off = s ? nW0 : 0;
+		double2 acc = make_double2(0., 0.);
+		for (uint32_t i = 0; i < nw; ++i) {
+			const double2 t = prod[((size_t)r * 2 + s) * wmax + i];
+			const double2 x = A[(size_t)ev * W + off + i];
+			acc.x = fma(t.x, x.x, acc.x);
+			acc.x = fma(-t.y, x.y, acc.x);
+			acc.y = fma(t.x, x.y, acc.y);
+			acc.y = fma(t.y, x.x, acc.y);
+		}
+		Sx[idx] = acc;
+	}
+	__syncthreads();
+	if (threadIdx.x < ne) {
+		const uint32_t ev = threadIdx.x;
+		double sigma = 0.;
+		for (uint32_t rs = 0; rs < R2; ++rs) {
+			const double2 v = Sx[(size_t)ev * R2 + rs];
+			sigma += v.x * v.x + v.y * v.y;
+		}
+		sigma += flat2;
+		const double factor = 2. / sigma;
+		wD[k0 + ev] = factor * factor;
+		wA[k0 + ev] = -factor;
+	}
+	// d rows (entry q = r W + compact wave g; d = S conj(A)) and the constant column
+	const uint32_t nD = nd + 1;
+	for (uint32_t idx = threadIdx.x; idx < ne * nD; idx += blockDim.x) {
+		const uint32_t ev = idx / nD, q = idx - ev * nD;
+		double2 v = make_double2(1., 0.);
+		if (q < nd) {
+			const uint32_t r = q / W, g = q - r * W;
+			const double2 S = Sx[(size_t)ev * R2 + 2 * r + (g >= nW0 ? 1 : 0)];
+			const double2 x = A[(size_t)ev * W + g];
+			v = make_double2(S.x * x.x + S.y * x.y, S.y * x.x - S.x * x.y);
+		}
+		dRows[(k0 + ev) * nD + q] = v;
+	}
+	const uint32_t nA = W + 1;
+	for (uint32_t idx = threadIdx.x; idx < ne * nA; idx += blockDim.x) {
+		const uint32_t ev = idx / nA, g = idx - ev * nA;
+		aRows[(k0 + ev) * nA + g] = g < W ? A[(size_t)ev * W + g] : make_double2(1., 0.);
+	}
 }
 
 }  // namespace
@@ -132,18 +150,22 @@ extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint3
 			return RPWA_ERR_NOMEM;
 		}
 		hipError_t e = hipMemcpy(prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2), hipMemcpyHostToDevice);
+		// events per block of the row kernel: their rows and S within 48 KB of LDS
+		const size_t perEvent = ((size_t)W + 2 * rank) * sizeof(double2);
+		const uint32_t nEv = (uint32_t)std::max<size_t>(1, std::min<size_t>(16, (48u << 10) / perEvent));
+		const size_t ldsBytes = (size_t)nEv * perEvent;
 		std::vector<double> CD((size_t)dD * dD, 0.), CA((size_t)dA * dA, 0.), part;
 		int st = RPWA_OK;
 		for (uint64_t e0 = 0; e == hipSuccess && st == RPWA_OK && e0 < lay.nEvents; e0 += cn) {
 			const uint64_t n = std::min<uint64_t>(cn, lay.nEvents - e0);
-			const uint32_t nb = (uint32_t)((n + 255) / 256);
+			const uint32_t nb = (uint32_t)((n + nEv - 1) / nEv);
 			if (lay.precision == RPWA_PREC_MIXED)
-				hipLaunchKernelGGL(hessian_rows_kernel<true>, dim3(nb), dim3(256), 0, 0, ctx->d_amps, e0, n,
-				                   lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
+				hipLaunchKernelGGL(hessian_rows_kernel<true>, dim3(nb), dim3(256), ldsBytes, 0, ctx->d_amps, e0, n,
+				                   nEv, lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
 				                   prodAmpFlat * prodAmpFlat, nd, dRows, aRows, wD, wA);
 			else
-				hipLaunchKernelGGL(hessian_rows_kernel<false>, dim3(nb), dim3(256), 0, 0, ctx->d_amps, e0, n,
-				                   lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
+				hipLaunchKernelGGL(hessian_rows_kernel<false>, dim3(nb), dim3(256), ldsBytes, 0, ctx->d_amps, e0, n,
+				                   nEv, lay.rowLen, lay.secOff[1], lay.nW[0], lay.nW[1], wmax, (int)rank, prod,
 				                   prodAmpFlat * prodAmpFlat, nd, dRows, aRows, wD, wA);
 			e = hipGetLastError();
 			if (e == hipSuccess)
