@@ -133,8 +133,8 @@ extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint3
 		RPWA_HIP_CHECK(hipSetDevice(ctx->device));
 		const uint32_t nd = rank * W;  // valid (rank, sector, wave) entries
 		const uint32_t dD = 2 * (nd + 1), dA = 2 * (W + 1);
-		// event chunk: row buffers of at most ~2 GiB
-		const uint64_t chunk = std::max<uint64_t>(1024, (2ull << 30) / (16ull * (nd + 1)));
+		// event chunk: row buffers of at most ~8 GiB (2M events x 345 parameters in two chunks)
+		const uint64_t chunk = std::max<uint64_t>(1024, (8ull << 30) / (16ull * (nd + 1)));
 		const uint64_t cn = std::min<uint64_t>(chunk, lay.nEvents);
 		double2 *dRows = nullptr, *aRows = nullptr, *prod = nullptr;
 		double *wD = nullptr, *wA = nullptr;
