@@ -719,24 +719,32 @@ struct BatchPlanArgs {
 
 // one block: per bin the evaluated starts packed into the leading columns (in
 // start order), the column groups that hold them, and the work units of the
-// next pass in (bin, chunk, group) order; the number of running fits
+// next pass in (bin, chunk, group) order; the number of running fits.  A wave
+// handles a bin with one lane per start (the packing is a ballot and a
+// popcount), the unit offsets are a block-wide scan over the bins.
 __global__ __launch_bounds__(256) void batch_plan_kernel(const BatchPlanArgs a)
 {
+	constexpr int kPlanWaves = 256 / 64;
 	__shared__ int32_t red[256];
+	__shared__ uint32_t scan[256];
+	const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	int32_t count = 0;
-	for (uint32_t b = threadIdx.x; b < a.nBins; b += 256) {
-		int k = 0;
-		for (int s = 0; s < kCols; ++s) {
-			const bool run = a.evalAll ? (uint32_t)s < a.nStarts : running(a.status[(size_t)b * kCols + s]);
-			a.startCol[(size_t)b * kCols + s] = run ? k : -1;
-			if (run)
-				a.colMap[(size_t)b * kCols + k++] = s;
+	for (uint32_t b = wv; b < a.nBins; b += kPlanWaves) {
+		const uint32_t s = lane;
+		const bool run = a.evalAll ? s < a.nStarts : running(a.status[(size_t)b * kCols + s]);
+		const uint64_t mask = __ballot(run);
+		const int k = __popcll(mask & ((1ull << lane) - 1ull));  // runs among the starts before s
+		const int nRun = __popcll(mask);
+		a.startCol[(size_t)b * kCols + s] = run ? k : -1;
+		if (run)
+			a.colMap[(size_t)b * kCols + k] = (int32_t)s;
+		if ((int)lane >= nRun)
+			a.colMap[(size_t)b * kCols + lane] = -1;
+		if (lane == 0) {
+			const uint32_t groups = (uint32_t)(nRun + kGroup - 1) / kGroup;
+			a.unitCount[b] = groups * (a.binChunk0[b + 1] - a.binChunk0[b]);
+			count += a.evalAll ? 0 : nRun;
 		}
-		for (int c = k; c < kCols; ++c)
-			a.colMap[(size_t)b * kCols + c] = -1;
-		const uint32_t groups = (uint32_t)(k + kGroup - 1) / kGroup;
-		a.unitCount[b] = groups * (a.binChunk0[b + 1] - a.binChunk0[b]);
-		count += a.evalAll ? 0 : k;
 	}
 	red[threadIdx.x] = count;
 	__syncthreads();
@@ -747,25 +755,34 @@ __global__ __launch_bounds__(256) void batch_plan_kernel(const BatchPlanArgs a)
 	}
 	if (threadIdx.x == 0)
 		*a.nRunning = red[0];
-	// unit offsets per bin (sequential: bins are few), then the units
-	__shared__ uint32_t total;
-	if (threadIdx.x == 0) {
-		uint32_t off = 0;
-		for (uint32_t b = 0; b < a.nBins; ++b) {
-			const uint32_t n = a.unitCount[b];
-			a.unitCount[b] = off;
-			off += n;
+	// exclusive scan of the unit counts over the bins, 256 bins at a time
+	uint32_t carry = 0;
+	for (uint32_t b0 = 0; b0 < a.nBins; b0 += 256) {
+		const uint32_t b = b0 + threadIdx.x;
+		const uint32_t n = b < a.nBins ? a.unitCount[b] : 0u;
+		scan[threadIdx.x] = n;
+		__syncthreads();
+		for (uint32_t d = 1; d < 256; d <<= 1) {  // inclusive Hillis-Steele scan
+			const uint32_t v = threadIdx.x >= d ? scan[threadIdx.x - d] : 0u;
+			__syncthreads();
+			scan[threadIdx.x] += v;
+			__syncthreads();
 		}
-		total = off;
-		*a.nUnits = off;
+		if (b < a.nBins)
+			a.unitCount[b] = carry + scan[threadIdx.x] - n;  // offset of bin b
+		carry += scan[255];
+		__syncthreads();
 	}
+	if (threadIdx.x == 0)
+		*a.nUnits = carry;
 	__syncthreads();
-	for (uint32_t b = 0; b < a.nBins; ++b) {
+	// the units of bin b (a wave per bin): chunk-major, column group minor
+	for (uint32_t b = wv; b < a.nBins; b += kPlanWaves) {
 		const uint32_t off = a.unitCount[b];
-		const uint32_t n = (b + 1 < a.nBins ? a.unitCount[b + 1] : total) - off;
+		const uint32_t n = (b + 1 < a.nBins ? a.unitCount[b + 1] : carry) - off;
 		const uint32_t chunks = a.binChunk0[b + 1] - a.binChunk0[b];
 		const uint32_t groups = chunks ? n / chunks : 0;
-		for (uint32_t k = threadIdx.x; k < n; k += 256) {
+		for (uint32_t k = lane; k < n; k += 64) {
 			const uint32_t c = k / groups, g = k % groups;
 			a.units[off + k] = (a.binChunk0[b] + c) * 4 + g;
 		}
