@@ -606,24 +606,37 @@ __global__ __launch_bounds__(64) void batch_lbfgs_kernel(const BatchLbfgsArgs a)
 			const uint32_t k = lane + 64 * r;
 			q[r] = k < P ? g[k] : 0.;
 		}
+		// the stored pairs in registers, loaded together (the recursion below is a
+		// chain of dependent dot products; no load waits inside it)
+		double sreg[kMaxMem][kPer], yreg[kMaxMem][kPer], rreg[kMaxMem];
+#pragma unroll
+		for (int c = 0; c < kMaxMem; ++c) {
+			const int slot = (head - 1 - c + 2 * (int)m) % (int)m;
+			const bool use = c < count;
+			rreg[c] = use ? rho[slot] : 0.;
+#pragma unroll
+			for (int r = 0; r < kPer; ++r) {
+				const uint32_t k = lane + 64 * r;
+				const bool in = use && k < P;
+				sreg[c][r] = in ? Sm[(size_t)slot * P + k] : 0.;
+				yreg[c][r] = in ? Ym[(size_t)slot * P + k] : 0.;
+			}
+		}
 		double al[kMaxMem];
 #pragma unroll
 		for (int c = 0; c < kMaxMem; ++c) {
 			if (c < count) {
-				const int slot = (head - 1 - c + 2 * (int)m) % (int)m;
-				const double* sv = Sm + (size_t)slot * P;
-				const double* yv = Ym + (size_t)slot * P;
 				double s = 0.;
 				for (int r = 0; r < kPer; ++r) {
 					const uint32_t k = lane + 64 * r;
 					if (k < P)
-						s += sv[k] * q[r];
+						s += sreg[c][r] * q[r];
 				}
-				al[c] = rho[slot] * wave_sum(s);
+				al[c] = rreg[c] * wave_sum(s);
 				for (int r = 0; r < kPer; ++r) {
 					const uint32_t k = lane + 64 * r;
 					if (k < P)
-						q[r] -= al[c] * yv[k];
+						q[r] -= al[c] * yreg[c][r];
 				}
 			}
 		}
@@ -633,20 +646,17 @@ __global__ __launch_bounds__(64) void batch_lbfgs_kernel(const BatchLbfgsArgs a)
 #pragma unroll
 		for (int c = kMaxMem - 1; c >= 0; --c) {
 			if (c < count) {
-				const int slot = (head - 1 - c + 2 * (int)m) % (int)m;
-				const double* sv = Sm + (size_t)slot * P;
-				const double* yv = Ym + (size_t)slot * P;
 				double s = 0.;
 				for (int r = 0; r < kPer; ++r) {
 					const uint32_t k = lane + 64 * r;
 					if (k < P)
-						s += yv[k] * q[r];
+						s += yreg[c][r] * q[r];
 				}
-				const double beta = rho[slot] * wave_sum(s);
+				const double beta = rreg[c] * wave_sum(s);
 				for (int r = 0; r < kPer; ++r) {
 					const uint32_t k = lane + 64 * r;
 					if (k < P)
-						q[r] += sv[k] * (al[c] - beta);
+						q[r] += sreg[c][r] * (al[c] - beta);
 				}
 			}
 		}
