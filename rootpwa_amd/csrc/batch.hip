@@ -330,12 +330,14 @@ struct BatchReduceArgs {
 };
 
 // per fit: gradient partials in chunk order, norm term, value and gradient.
-// 256 threads: thread (h, i) adds half h of the bin's chunks for wave component
-// i (the halves are then added in order 0, 1), and the norm term's row sums
-// read T from LDS with the loads of eight columns issued together.
-constexpr int kRedThreads = 256;
-constexpr int kRedHalf = kRedThreads / 2;
-static_assert(RPWA_BATCH_MAX_WAVES <= kRedHalf, "one thread per wave component and half");
+// kRedParts x 128 threads: thread (h, i) adds part h of the bin's chunks for
+// wave component i (the parts are then added in order 0, 1, ...), and the norm
+// term's row sums read T from LDS with the loads of eight columns issued
+// together.
+constexpr int kRedParts = 4;
+constexpr int kRedHalf = 128;
+constexpr int kRedThreads = kRedParts * kRedHalf;
+static_assert(RPWA_BATCH_MAX_WAVES <= kRedHalf, "one thread per wave component and part");
 
 __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchReduceArgs r)
 {
@@ -345,12 +347,11 @@ __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchRe
 		return;
 	const uint32_t q = (uint32_t)col / kGroup, jc = (uint32_t)col % kGroup;
 	const uint32_t c0 = r.binChunk0[b], c1 = r.binChunk0[b + 1];
-	const uint32_t cm = c0 + (c1 - c0 + 1) / 2;
 	const size_t fit = blockIdx.x;
 	const uint32_t tid = threadIdx.x;
 	const uint32_t i = tid % kRedHalf, h = tid / kRedHalf;
 	__shared__ double red[3][kRedThreads];
-	__shared__ double2 gh[kRedHalf];   // the second half's gradient sums
+	__shared__ double2 gh[kRedParts - 1][kRedHalf];   // the later parts' gradient sums
 	__shared__ double2 Ts[kRedHalf];   // T of the fit
 	const float* Tf = r.T32 + fit * r.rowLen;
 	if (tid < r.W)
@@ -360,7 +361,9 @@ __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchRe
 		// chunk partials of wave component i, added in chunk order; the loads of
 		// eight chunks are issued before their adds
 		const size_t cstride = (size_t)4 * r.Wp * 2 * kGroup;  // floats between consecutive chunks
-		const uint32_t ca = h ? cm : c0, cb = h ? c1 : cm;
+		const uint32_t nc = c1 - c0;
+		const uint32_t ca = c0 + (uint32_t)(((uint64_t)nc * h) / kRedParts);
+		const uint32_t cb = c0 + (uint32_t)(((uint64_t)nc * (h + 1)) / kRedParts);
 		const float* pg = r.partG + ((size_t)(ca * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
 		uint32_t c = ca;
 		for (; c + 8 <= cb; c += 8, pg += 8 * cstride) {
@@ -379,9 +382,9 @@ __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchRe
 			gim += (double)pg[1];
 		}
 		if (h)
-			gh[i] = make_double2(gre, gim);
+			gh[h - 1][i] = make_double2(gre, gim);
 	}
-	// the per-chunk log sums and weight sums: thread t takes chunks c0+t, c0+t+256, ...
+	// the per-chunk log sums and weight sums: thread t takes chunks c0+t, c0+t+kRedThreads, ...
 	double lnPart = 0., invPart = 0.;
 	for (uint32_t c = c0 + tid; c < c1; c += kRedThreads) {
 		lnPart += r.partLn[(size_t)(c * 4 + q) * kGroup + jc];
@@ -390,8 +393,11 @@ __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchRe
 	__syncthreads();
 	double normPart = 0.;
 	if (h == 0 && i < r.W) {
-		gre += gh[i].x;
-		gim += gh[i].y;
+#pragma unroll
+		for (int part = 0; part < kRedParts - 1; ++part) {
+			gre += gh[part][i].x;
+			gim += gh[part][i].y;
+		}
 		// u_i = sum_j T_j conj(I_ij); norm = sum_i Re(T_i conj(u_i)); d norm / d T_i = 2 u_i
 		const double2* I = reinterpret_cast<const double2*>(r.acc) + ((size_t)b * r.W + i) * r.W;
 		double ure = 0., uim = 0.;
