@@ -332,7 +332,6 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 	// ---- workgroup partial row ----
 	lds_barrier();
 	double2* gbuf = reinterpret_cast<double2*>(smem);  // [gSubs][rowUnits][P][R]
-	double* scal = reinterpret_cast<double*>(gbuf + (size_t)a.gSubs * a.rowUnits * P * R);  // [E][3]
 	if (gOn) {
 #pragma unroll
 		for (int j = 0; j < UG; ++j) {
@@ -347,24 +346,33 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 			}
 		}
 	}
-	if (wid == 0 && (E == 64 || lane < E)) {
-		scal[lane * 3 + 0] = -(log(mant) + (double)expo * 0.69314718055994530942);
-		scal[lane * 3 + 1] = 0.;
-		scal[lane * 3 + 2] = wsum;
+	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
+	if (wid == 0) {
+		// the lanes' -ln sigma sums and weight sums, combined by a fixed butterfly:
+		// at each step the lower lane's value is the left operand in both partners,
+		// and the rounding error of each addition (TwoSum) goes to the compensation
+		const bool ev = E == 64 || lane < E;
+		double s = ev ? -(log(mant) + (double)expo * 0.69314718055994530942) : 0.;
+		double c = 0., ws = ev ? wsum : 0.;
+#pragma unroll
+		for (int m = 32; m >= 1; m >>= 1) {
+			const double so = __shfl_xor(s, m), co = __shfl_xor(c, m), wo = __shfl_xor(ws, m);
+			const bool upper = (lane & m) != 0;
+			const double x = upper ? so : s, y = upper ? s : so;
+			const double t = x + y;
+			const double yp = t - x;
+			const double err = (x - (t - yp)) + (y - yp);
+			c = (upper ? co + c : c + co) + err;
+			ws = upper ? wo + ws : ws + wo;
+			s = t;
+		}
+		if (lane == 0) {
+			out[0] = s;
+			out[1] = c;
+			out[2] = ws;
+		}
 	}
 	lds_barrier();
-	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
-	if (tid == 0) {
-		double s = 0., c = 0., ws = 0.;
-		for (int e = 0; e < E; ++e) {
-			ek::neumaier_add(s, c, scal[e * 3 + 0]);
-			ek::neumaier_add(s, c, scal[e * 3 + 1]);
-			ws += scal[e * 3 + 2];
-		}
-		out[0] = s;
-		out[1] = c;
-		out[2] = ws;
-	}
 	// element 3 + 2*((r*2 + s)*wmax + j) (+1 for the imaginary part)
 	const uint32_t nCplx = (a.nElem - 3) / 2;
 	for (uint32_t idx = tid; idx < nCplx; idx += kThreads) {
