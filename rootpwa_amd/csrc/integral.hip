@@ -63,8 +63,8 @@ __device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, ui
 // NT tiles per wave: wave w owns the contiguous block [w*NT, (w+1)*NT) of the
 // pass; slots past the end of the pass compute a dummy tile that is discarded,
 // so the MFMA loop has no data-dependent exits and the accumulators stay in
-// registers.
-template <int NT, int kChunk>
+// registers.  WEIGHTED = false: every weight is 1 (no multiply per MFMA operand).
+template <int NT, int kChunk, bool WEIGHTED>
 __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 {
 	extern __shared__ double lds[];   // 2 x ([kChunk][ldsStride] + weights[kChunk])
@@ -154,11 +154,11 @@ __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 #pragma unroll
 		for (int ks = 0; ks < kChunk / 4; ++ks) {
 			const int row = ks * 4 + lk;
-			const double wt = wts[row];
+			const double wt = WEIGHTED ? wts[row] : 1.;
 			const double* xr = xs + (size_t)row * a.ldsStride;
 #pragma unroll
 			for (int q = 0; q < NT; ++q) {
-				const double av = xr[tI[q]] * wt;
+				const double av = WEIGHTED ? xr[tI[q]] * wt : xr[tI[q]];
 				const double bv = xr[tJ[q]];
 				acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
 			}
@@ -260,24 +260,30 @@ static int integral_plan(int device, uint32_t D, IntegralPlan& p)
 }
 
 typedef void (*IntKernel)(IntArgs);
-template <int CH>
+template <int CH, bool WT>
 static IntKernel pick_int_nt(uint32_t per)
 {
-	if (per <= 1) return integral_kernel<1, CH>;
-	if (per <= 2) return integral_kernel<2, CH>;
-	if (per <= 4) return integral_kernel<4, CH>;
-	if (per <= 6) return integral_kernel<6, CH>;
-	if (per <= 9) return integral_kernel<9, CH>;
-	return integral_kernel<12, CH>;
+	if (per <= 1) return integral_kernel<1, CH, WT>;
+	if (per <= 2) return integral_kernel<2, CH, WT>;
+	if (per <= 4) return integral_kernel<4, CH, WT>;
+	if (per <= 6) return integral_kernel<6, CH, WT>;
+	if (per <= 9) return integral_kernel<9, CH, WT>;
+	return integral_kernel<12, CH, WT>;
 }
 
-static IntKernel pick_int_kernel(uint32_t nTiles, int chunk)
+template <bool WT>
+static IntKernel pick_int_chunk(uint32_t per, int chunk)
+{
+	if (chunk >= 32) return pick_int_nt<32, WT>(per);
+	if (chunk >= 16) return pick_int_nt<16, WT>(per);
+	if (chunk >= 8) return pick_int_nt<8, WT>(per);
+	return pick_int_nt<4, WT>(per);
+}
+
+static IntKernel pick_int_kernel(uint32_t nTiles, int chunk, bool weighted)
 {
 	const uint32_t per = (nTiles + kIntWaves - 1) / kIntWaves;
-	if (chunk >= 32) return pick_int_nt<32>(per);
-	if (chunk >= 16) return pick_int_nt<16>(per);
-	if (chunk >= 8) return pick_int_nt<8>(per);
-	return pick_int_nt<4>(per);
+	return weighted ? pick_int_chunk<true>(per, chunk) : pick_int_chunk<false>(per, chunk);
 }
 
 static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_t r, uint32_t c)
@@ -329,7 +335,7 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	INT_CHECK(hipMalloc(&d_part, partN * sizeof(double)));
 	INT_CHECK(hipMalloc(&d_wsum, p.nWg * sizeof(double)));
 	INT_CHECK(hipMalloc(&d_red, (size_t)p.nTiles * 256 * sizeof(double)));
-	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk);
+	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
 	// every pass re-reads the rows for its own set of output tiles
 	auto launch_passes = [&](IntArgs aa) -> hipError_t {
 		for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
