@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		const uint32_t row = g / a.rowUnits;
 		const bool in = g < a.tileUnits;
 		ldsPos[k] = in ? row * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
-		ldOff[k] = (in ? g : (uint32_t)tid) * 16u;  // byte offset in the tile
+		ldOff[k] = in ? g * 16u : 0u;  // byte offset in the tile (a load past the tile reads its first unit)
 	}
 
 	// G-phase role: unit group gi of sector gSec, units [gU0, min(gU0 + UG, gEnd))
@@ -284,8 +284,9 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 	int expo = 0;
 
 	// Every thread issues kMaxLoads loads per tile, so that the staging registers
-	// are a fixed set: a load past the tile repeats the thread's first unit, and
-	// indices past the array are clamped to its last unit (rows past the last
+	// are a fixed set: a load past the tile reads the tile's first unit (the
+	// tiles before the last are complete, so it stays inside the array), and in
+	// the last tile indices past the array are clamped to its last unit (rows past the last
 	// event hold copies of real amplitudes and get weight zero).  The loads of
 	// the next tile are in flight while a tile is computed (the fetch after the
 	// block's last tile re-reads the array's last tile, so that every stage

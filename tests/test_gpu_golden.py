@@ -81,3 +81,27 @@ def test_ragged_event_counts_every_variant(gpu, nEvents):
             assert abs(f - f_ref) <= tol_f * max(1.0, abs(f_ref))
             assert np.max(np.abs(d - d_ref)) <= tol_d * np.max(np.abs(d_ref))
             assert abs(df - df_ref) <= tol_d * max(1.0, abs(df_ref))
+
+
+@pytest.mark.parametrize("nW", [(1, 1), (1, 2), (2, 3)])
+@pytest.mark.parametrize("nEvents", [1, 64, 65, 200, 4097])
+def test_small_rows_staged_kernel(gpu, nW, nEvents):
+    """Rows of 2-5 16-byte units: a staged tile has fewer units than a
+    workgroup has threads, so most threads' loads fall past the tile and must
+    read inside it (its first unit) rather than past the amplitude array."""
+    rank = 1 if nW == (1, 1) else 2
+    ds = H.dataset(nW, rank, nData=nEvents, nMC=1000, seed=700 + nEvents + 10 * sum(nW))
+    L = H.oracle_likelihood(ds)
+    amps = H.normalised_amps(L)
+    prod = H.random_prod(np.random.default_rng(nEvents + 1), rank, nW, 0.8)
+    f_ref = L.event_loglik(prod, 0.3)
+    d_ref, df_ref = L.event_deriv(prod, 0.3)
+    for prec, tol_f, tol_d in ((0, 1e-12, 1e-11), (1, 1e-6, 1e-5)):
+        ctx = gpu.LikelihoodContext(0, prec)
+        ctx.load(amps, nW)
+        ctx.set_tile(1)
+        print("small rows", nW, "events", nEvents, "precision", prec, flush=True)
+        f, d, df = ctx.fdf(prod, 0.3)
+        assert abs(f - f_ref) <= tol_f * max(1.0, abs(f_ref))
+        assert np.max(np.abs(d - d_ref)) <= tol_d * np.max(np.abs(d_ref))
+        assert abs(df - df_ref) <= tol_d * max(1.0, abs(df_ref))
