@@ -139,9 +139,10 @@ int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
  * results of a rank always come from one variant.  Variants:
  * lanes per event in {8,16,32,64} (register tiles: a group of lanes holds one
  * event's amplitude row and the production amplitudes of its columns) and
- * lanes per event 1 (LDS-staged: tiles of 16..64 events pass through LDS, one
- * lane contracts one event, production amplitudes come through the scalar
- * unit).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
+ * lanes per event 1 (LDS-staged: tiles of 64, 32 or 16 events, each timed,
+ * pass through LDS, one lane contracts one event, production amplitudes come
+ * through the scalar unit; rpwa_lik_set_tile(ctx, 1) takes the largest tile
+ * that fits).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
  * environment) keeps the modelled default instead; rpwa_lik_set_tile fixes the
  * variant of every rank (RPWA_ERR_UNSUPPORTED if the row does not fit it). */
 int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable);

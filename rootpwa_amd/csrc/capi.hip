@@ -62,12 +62,13 @@ bool make_tile(const AmpLayout& lay, int lanes, TileShape& ts)
 	return false;
 }
 
-// Tile of the LDS-staged kernel: the largest feasible number of events per tile.
-bool make_stage_tile(const AmpLayout& lay, int rank, TileShape& ts)
+// Tile of the LDS-staged kernel: the largest feasible number of events per
+// tile, at most maxEvents.
+bool make_stage_tile(const AmpLayout& lay, int rank, TileShape& ts, int maxEvents = 64)
 {
 	for (int e : {64, 32, 16}) {
 		StagePlan p;
-		if (stage_plan(lay, e, rank, p)) {
+		if (e <= maxEvents && stage_plan(lay, e, rank, p)) {
 			ts = TileShape();
 			ts.lanes = 1;
 			ts.slots = e;
@@ -268,9 +269,13 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	TileShape best = keep;
 	double bestT = 1e300;
 	const int reps = 5;
-	for (int lanes : {8, 16, 32, 64, 1}) {
+	// register tiles of 8..64 lanes per event, then the LDS-staged kernel with
+	// tiles of 64, 32 and 16 events (the smaller tiles give a short event list
+	// more tiles per workgroup)
+	for (int cand : {8, 16, 32, 64, -64, -32, -16}) {
 			TileShape ts;
-			if (lanes == 1 ? !make_stage_tile(c->lay, rank, ts) : !make_tile(c->lay, lanes, ts))
+			if (cand > 0 ? !make_tile(c->lay, cand, ts)
+			             : (!make_stage_tile(c->lay, rank, ts, -cand) || ts.events != -cand))
 				continue;
 			c->tile[rank] = ts;
 			if (int st = configure_grid(c, rank))
