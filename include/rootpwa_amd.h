@@ -147,6 +147,11 @@ int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
  * variant of every rank (RPWA_ERR_UNSUPPORTED if the row does not fit it). */
 int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable);
 int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
+/* Fixes the LDS-staged kernel with tiles of eventsPerTile (16, 32 or 64)
+ * events for every rank; RPWA_ERR_UNSUPPORTED if such a tile does not fit.
+ * No counterpart in the reference: a tuning and testing control like
+ * rpwa_lik_set_tile, for the tile sizes the autotuner chooses between. */
+int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile);
 
 /* Stream on which the context launches its kernels (for event timing). */
 int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream);

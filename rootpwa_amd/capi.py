@@ -56,6 +56,7 @@ PROTOTYPES = {
     "rpwa_lik_kernel_times": (_i, [_vp, _dp, _dp, _u64p]),
     "rpwa_lik_info": (_i, [_vp, _u64p]),
     "rpwa_lik_set_tile": (_i, [_vp, _u32]),
+    "rpwa_lik_set_stage_tile": (_i, [_vp, _u32]),
     "rpwa_lik_set_autotune": (_i, [_vp, _i]),
     "rpwa_lik_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
     "rpwa_dev_alloc": (_i, [_i, _u64, ctypes.POINTER(_vp)]),
@@ -237,6 +238,9 @@ class LikelihoodContext:
 
     def set_tile(self, lanes):
         call("rpwa_lik_set_tile", self.h, int(lanes))
+
+    def set_stage_tile(self, events):
+        call("rpwa_lik_set_stage_tile", self.h, int(events))
 
     def set_autotune(self, enable):
         call("rpwa_lik_set_autotune", self.h, int(bool(enable)))

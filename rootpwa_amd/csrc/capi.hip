@@ -750,6 +750,40 @@ extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
 	RPWA_GUARD_END
 }
 
+extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_set_stage_tile"))
+		return st;
+	if (!ctx->loaded) {
+		set_error("rpwa_lik_set_stage_tile: decay amplitudes not loaded");
+		return RPWA_ERR_STATE;
+	}
+	const int e = (int)eventsPerTile;
+	if (e != 16 && e != 32 && e != 64) {
+		set_error("rpwa_lik_set_stage_tile: events per tile must be 16, 32 or 64");
+		return RPWA_ERR_INVALID;
+	}
+	TileShape ts;
+	for (int r = 1; r <= kMaxRank; ++r)
+		if (!make_stage_tile(ctx->lay, r, ts, e) || ts.events != e) {
+			set_error("rpwa_lik_set_stage_tile: a tile of " + std::to_string(e) +
+			          " amplitude rows does not fit the LDS-staged kernel");
+			return RPWA_ERR_UNSUPPORTED;
+		}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
+	for (int r = 1; r <= kMaxRank; ++r) {
+		make_stage_tile(ctx->lay, r, ts, e);
+		ctx->tile[r] = ts;
+		ctx->tileFixed[r] = true;
+		if (int st = configure_grid(ctx, r))
+			return st;
+	}
+	return RPWA_OK;
+	RPWA_GUARD_END
+}
+
 extern "C" int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable)
 {
 	clear_error();
