@@ -72,11 +72,13 @@ int rpwa_lik_load_decay_amps(rpwa_lik_ctx* ctx, const double* decayAmps, uint64_
                              uint64_t nmbEvents, const uint32_t nmbWavesRefl[2], int layout);
 
 /* Same, from amplitudes already in device memory on the context's device, in
- * RPWA_LAYOUT_EVT_WAVE order.  waveScale (host, [W0+W1], may be NULL) multiplies
- * each wave's column in FP64 before the amplitudes are stored (the
- * 1/sqrt(Re I_ii) normalisation of pwaLikelihood.cc:1253-1260). */
+ * RPWA_LAYOUT_EVT_WAVE order.  waveNorm (host, [W0+W1], may be NULL) divides
+ * each wave's column (real and imaginary parts) in FP64 before the amplitudes
+ * are stored: with waveNorm = sqrt(Re I^norm_ii) it is the normalisation
+ * amps /= sqrt(normInt.real()) of pwaLikelihood.cc:1253-1260, with the same
+ * rounding as that host division. */
 int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* d_amps, uint64_t nmbEvents,
-                                    const uint32_t nmbWavesRefl[2], const double* waveScale);
+                                    const uint32_t nmbWavesRefl[2], const double* waveNorm);
 
 /* Event part of -lnL: sum_e -ln( sum_r sum_refl |sum_i T_r,refl,i A_refl,i(e)|^2 + flat^2 )
  * (replaces likelihoodInterface::logLikelihood, likelihoodInterface.cuh:112-115).
@@ -171,6 +173,15 @@ int rpwa_dev_synchronize(int device);
 /* name (NUL-terminated, truncated to nameLen), compute units, global memory
  * bytes, peak engine clock [kHz] */
 int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* totalMem, int* clockKhz);
+
+/* Whether the amplitudes of one wave (column `wave` of d_amps [nmbEvents][nmbWaves]
+ * complex128, device memory) are non-zero for any event: *nonzero = 1, else 0.
+ * The device-memory form of addAmplitude's zero-amplitude check
+ * (partialWaveFit/pwaLikelihood.cc:1247-1264), which fixes the production
+ * amplitudes of a wave whose acceptance integral and amplitudes are all zero
+ * and rejects non-zero amplitudes of a wave with a zero normalisation integral. */
+int rpwa_wave_nonzero_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                             uint32_t wave, int* nonzero);
 
 /* HBM bandwidth probe: median GB/s of a streaming read and of a copy (read +
  * write bytes) over `bytes` of device memory; the counterpart of the

@@ -150,9 +150,18 @@ public:
 	void setGpuDevice(const int device) { _device = device; }
 	void useNormalizedAmps(const bool useNorm = true) { _useNormalizedAmps = useNorm; }
 	bool normalizedAmpsUsed() const { return _useNormalizedAmps; }
-	void setPriorType(const priorEnum priorType = FLAT) { _priorType = priorType; }
+	// both change the objective: a value+gradient kept from an earlier DoEval no longer applies
+	void setPriorType(const priorEnum priorType = FLAT)
+	{
+		_priorType = priorType;
+		_evalCacheValid = false;
+	}
 	priorEnum priorType() const { return _priorType; }
-	void setCauchyWidth(const double cauchyWidth) { _cauchyWidth = cauchyWidth; }
+	void setCauchyWidth(const double cauchyWidth)
+	{
+		_cauchyWidth = cauchyWidth;
+		_evalCacheValid = false;
+	}
 	double cauchyWidth() const { return _cauchyWidth; }
 
 	// operations

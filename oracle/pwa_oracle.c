@@ -559,12 +559,18 @@ static void fdf_impl(orc_lik* L, const double* par, double* f, double* grad, int
 		}
 		for (int t = 0; t < nThreads; ++t)
 			pthread_join(th[t], NULL);
-		/* merge the range partials in range order */
+		/* merge the range partials in range order; a Kahan partial (s, c) stands
+		 * for s - c, so both the sum and its compensation are added */
 		for (int t = 0; t < nThreads; ++t) {
 			k_add(&acc.lnl, jobs[t].acc.lnl.s);
+			k_add(&acc.lnl, -jobs[t].acc.lnl.c);
 			k_add(&acc.dflat, jobs[t].acc.dflat.s);
-			for (size_t k = 0; k < nd; ++k)
-				kc_add(&acc.dacc[k], kc_sum(&jobs[t].acc.dacc[k]));
+			k_add(&acc.dflat, -jobs[t].acc.dflat.c);
+			for (size_t k = 0; k < nd; ++k) {
+				const kahanc* p = &jobs[t].acc.dacc[k];
+				kc_add(&acc.dacc[k], kc_sum(p));
+				kc_add(&acc.dacc[k], c_make(-p->re.c, -p->im.c));
+			}
 			free(jobs[t].acc.dacc);
 		}
 		free(jobs);

@@ -64,6 +64,7 @@ PROTOTYPES = {
     "rpwa_dev_copy": (_i, [_vp, _vp, _u64, _i]),
     "rpwa_dev_synchronize": (_i, [_i]),
     "rpwa_dev_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i), _u64p, ctypes.POINTER(_i)]),
+    "rpwa_wave_nonzero_device": (_i, [_i, _vp, _u64, _u32, _u32, ctypes.POINTER(_i)]),
     "rpwa_bandwidth_probe": (_i, [_i, _u64, _i, _dp, _dp]),
     "rpwa_integral": (_i, [_i, _dp, _u64, _u32, _dp, _dp]),
     "rpwa_integral_device": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
@@ -155,11 +156,11 @@ class LikelihoodContext:
         call("rpwa_lik_load_decay_amps", self.h, _cd(a), a.size, int(nEvents), nw, int(layout))
         self.nW = tuple(nW)
 
-    def load_device(self, d_ptr, nEvents, nW, waveScale=None):
+    def load_device(self, d_ptr, nEvents, nW, waveNorm=None):
         nw = (ctypes.c_uint32 * 2)(*nW)
         sc = None
-        if waveScale is not None:
-            self._scale = np.ascontiguousarray(waveScale, dtype=np.float64)
+        if waveNorm is not None:
+            self._scale = np.ascontiguousarray(waveNorm, dtype=np.float64)
             sc = _dptr(self._scale)
         call("rpwa_lik_load_decay_amps_device", self.h, _vp(d_ptr), int(nEvents), nw, sc)
         self.nW = tuple(nW)
@@ -298,6 +299,13 @@ def device_info(device=0):
     clk = ctypes.c_int()
     call("rpwa_dev_info", int(device), name, 64, ctypes.byref(cu), ctypes.byref(mem), ctypes.byref(clk))
     return dict(arch=name.value.decode(), cus=cu.value, memory=mem.value, clock_khz=clk.value)
+
+
+def wave_nonzero_device(d_amps, N, W, wave, device=0):
+    """True if column `wave` of the device array [N][W] complex128 has a non-zero element."""
+    out = _i()
+    call("rpwa_wave_nonzero_device", device, _vp(d_amps), N, W, wave, ctypes.byref(out))
+    return bool(out.value)
 
 
 def bandwidth_probe(nbytes=4 << 30, repetitions=10, device=0):

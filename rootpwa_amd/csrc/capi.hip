@@ -186,6 +186,7 @@ int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
 	for (int r = 0; r <= kMaxRank; ++r) {
 		c->tile[r] = ts;
 		c->tileFixed[r] = false;
+		c->evaluated[r] = false;
 	}
 	return RPWA_OK;
 }
@@ -208,7 +209,7 @@ int alloc_eval_buffers(rpwa_lik_ctx* c)
 
 // load from FP64 amplitudes already on the device
 int load_from_device(rpwa_lik_ctx* c, const double* d_src, int srcLayout, uint64_t nEvents,
-                     const uint32_t nW[2], const double* waveScale)
+                     const uint32_t nW[2], const double* waveNorm)
 {
 	free_buffers(c);
 	c->partialCap = 0;
@@ -216,16 +217,16 @@ int load_from_device(rpwa_lik_ctx* c, const double* d_src, int srcLayout, uint64
 	if (st != RPWA_OK)
 		return st;
 	RPWA_HIP_CHECK(hipMalloc(&c->d_amps, std::max<size_t>(c->lay.bytes(), 16)));
-	double* d_scale = nullptr;
-	if (waveScale) {
+	double* d_norm = nullptr;
+	if (waveNorm) {
 		const uint32_t W = nW[0] + nW[1];
-		RPWA_HIP_CHECK(hipMalloc(&d_scale, W * sizeof(double)));
-		RPWA_HIP_CHECK(hipMemcpy(d_scale, waveScale, W * sizeof(double), hipMemcpyHostToDevice));
+		RPWA_HIP_CHECK(hipMalloc(&d_norm, W * sizeof(double)));
+		RPWA_HIP_CHECK(hipMemcpy(d_norm, waveNorm, W * sizeof(double), hipMemcpyHostToDevice));
 	}
-	hipError_t e = launch_transform_amps(c->lay, d_src, srcLayout, d_scale, c->d_amps, c->stream);
+	hipError_t e = launch_transform_amps(c->lay, d_src, srcLayout, d_norm, c->d_amps, c->stream);
 	if (e == hipSuccess)
 		e = hipStreamSynchronize(c->stream);
-	hipFree(d_scale);
+	hipFree(d_norm);
 	RPWA_HIP_CHECK(e);
 	st = alloc_eval_buffers(c);
 	if (st != RPWA_OK)
@@ -382,6 +383,7 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, flat, c->d_out, c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
 	c->pending = true;
+	c->evaluated[rank] = true;
 	c->pendingRank = rank;
 	c->pendingProd = nmbProdAmps;
 	return RPWA_OK;
@@ -563,7 +565,7 @@ extern "C" int rpwa_lik_load_decay_amps(rpwa_lik_ctx* ctx, const double* decayAm
 }
 
 extern "C" int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* d_amps, uint64_t nmbEvents,
-                                               const uint32_t nmbWavesRefl[2], const double* waveScale)
+                                               const uint32_t nmbWavesRefl[2], const double* waveNorm)
 {
 	clear_error();
 	if (int st = check_ctx(ctx, "rpwa_lik_load_decay_amps_device"))
@@ -574,7 +576,7 @@ extern "C" int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* 
 	}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
-	return load_from_device(ctx, d_amps, RPWA_LAYOUT_EVT_WAVE, nmbEvents, nmbWavesRefl, waveScale);
+	return load_from_device(ctx, d_amps, RPWA_LAYOUT_EVT_WAVE, nmbEvents, nmbWavesRefl, waveNorm);
 	RPWA_GUARD_END
 }
 
@@ -704,7 +706,7 @@ extern "C" int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9])
 	info[3] = (uint64_t)ctx->precision;
 	int r = 1;
 	for (int k = kMaxRank; k >= 1; --k)
-		if (ctx->tileFixed[k])
+		if (ctx->evaluated[k])
 			r = k;  // lowest rank evaluated so far (1 if none)
 	info[4] = (uint64_t)ctx->tile[r].lanes;
 	info[5] = (uint64_t)ctx->tile[r].slots;
@@ -856,6 +858,33 @@ extern "C" int rpwa_dev_synchronize(int device)
 	RPWA_HIP_CHECK(hipSetDevice(device));
 	RPWA_HIP_CHECK(hipDeviceSynchronize());
 	return RPWA_OK;
+}
+
+extern "C" int rpwa_wave_nonzero_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                                        uint32_t wave, int* nonzero)
+{
+	clear_error();
+	if (!d_amps || !nonzero || nmbEvents == 0 || wave >= nmbWaves) {
+		set_error("rpwa_wave_nonzero_device: null pointer, no events or wave index outside the row");
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_GUARD_BEGIN
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	int nCu = 0;
+	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
+	unsigned int* d_flag = nullptr;
+	RPWA_HIP_CHECK(hipMalloc(&d_flag, sizeof(unsigned int)));
+	unsigned int flag = 0;
+	hipError_t e = hipMemcpy(d_flag, &flag, sizeof(flag), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = launch_wave_nonzero(d_amps, nmbEvents, nmbWaves, wave, d_flag, nCu, nullptr);
+	if (e == hipSuccess)
+		e = hipMemcpy(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost);
+	hipFree(d_flag);
+	RPWA_HIP_CHECK(e);
+	*nonzero = flag ? 1 : 0;
+	return RPWA_OK;
+	RPWA_GUARD_END
 }
 
 extern "C" int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* totalMem, int* clockKhz)

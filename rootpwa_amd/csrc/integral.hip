@@ -336,6 +336,10 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	INT_CHECK(hipMalloc(&d_wsum, p.nWg * sizeof(double)));
 	INT_CHECK(hipMalloc(&d_red, (size_t)p.nTiles * 256 * sizeof(double)));
 	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
+	// dynamic LDS beyond 64 KiB (the two chunk buffers at wide rows) is requested explicitly
+	if (p.lds + 32 * sizeof(double) > 64u * 1024u)
+		INT_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+		                              (int)p.lds));
 	// every pass re-reads the rows for its own set of output tiles
 	auto launch_passes = [&](IntArgs aa) -> hipError_t {
 		for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {

@@ -103,8 +103,11 @@ hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, 
 // weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T
 int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
              const double* h_x, const double* h_w, std::vector<double>& full, double* wtot);
+// *flag |= 1 if column `wave` of amps [nEvents][W] (complex128) has a non-zero element
+hipError_t launch_wave_nonzero(const double* d_amps, uint64_t nEvents, uint32_t W, uint32_t wave,
+                               unsigned int* flag, int nCu, hipStream_t s);
 hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int srcLayout,
-                                 const double* d_scale, void* d_dst, hipStream_t s);
+                                 const double* d_norm, void* d_dst, hipStream_t s);
 
 }  // namespace rpwa_amd
 
@@ -120,6 +123,7 @@ struct rpwa_lik_ctx {
 	rpwa_amd::TileShape tile[rpwa_amd::kMaxRank + 1];
 	int nWg[rpwa_amd::kMaxRank + 1] = {0, 0, 0, 0, 0};
 	bool tileFixed[rpwa_amd::kMaxRank + 1] = {false, false, false, false, false};
+	bool evaluated[rpwa_amd::kMaxRank + 1] = {false, false, false, false, false};
 	bool autotune = true;
 	bool loaded = false;
 	int nCu = 0;

@@ -28,6 +28,8 @@
 // No floating-point atomics are used.
 #include "event_kernel.h"
 
+#include <algorithm>
+
 namespace rpwa_amd {
 namespace {
 
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
 template <bool MIXED>
 __global__ void transform_kernel(const double2* __restrict__ src, int srcLayout, uint64_t nEvents,
                                  uint32_t nW0, uint32_t nW1, uint32_t wmax, uint32_t secOff1,
-                                 uint32_t rowLen, const double* __restrict__ scale, void* dst)
+                                 uint32_t rowLen, const double* __restrict__ norm, void* dst)
 {
 	const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= nEvents * rowLen)
@@ -150,16 +152,33 @@ __global__ void transform_kernel(const double2* __restrict__ src, int srcLayout,
 			v = src[e * (nW0 + nW1) + gw];
 		else
 			v = src[((uint64_t)s * wmax + j) * nEvents + e];
-		if (scale) {
-			const double f = scale[gw];
-			v.x *= f;
-			v.y *= f;
+		if (norm) {
+			// complex / real component-wise, correctly rounded: the host's a / sqrt(I_ii)
+			const double f = norm[gw];
+			v.x = v.x / f;
+			v.y = v.y / f;
 		}
 	}
 	if (MIXED)
 		reinterpret_cast<float2*>(dst)[idx] = make_float2((float)v.x, (float)v.y);
 	else
 		reinterpret_cast<double2*>(dst)[idx] = v;
+}
+
+// One wave's column of [nEvents][W] complex128 amplitudes: is any element
+// non-zero (NaN counts as non-zero, as `a != 0` does)?  One integer OR per
+// workgroup.
+__global__ __launch_bounds__(256) void wave_nonzero_kernel(const double2* __restrict__ amps, uint64_t nEvents,
+                                                           uint32_t W, uint32_t wave, unsigned int* flag)
+{
+	int nz = 0;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nEvents; e += stride) {
+		const double2 v = amps[e * W + wave];
+		nz |= (v.x != 0. || v.y != 0.) ? 1 : 0;
+	}
+	if (__syncthreads_or(nz) && threadIdx.x == 0)
+		atomicOr(flag, 1u);
 }
 
 EventKernel pick_kernel(const AmpLayout& lay, const TileShape& ts, int rank)
@@ -214,8 +233,18 @@ hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, 
 	return hipGetLastError();
 }
 
+hipError_t launch_wave_nonzero(const double* d_amps, uint64_t nEvents, uint32_t W, uint32_t wave,
+                               unsigned int* flag, int nCu, hipStream_t s)
+{
+	const uint64_t need = (nEvents + 255) / 256;
+	const uint32_t nb = (uint32_t)std::min<uint64_t>(need, (uint64_t)std::max(nCu, 1) * 8);
+	hipLaunchKernelGGL(wave_nonzero_kernel, dim3(std::max(nb, 1u)), dim3(256), 0, s,
+	                   reinterpret_cast<const double2*>(d_amps), nEvents, W, wave, flag);
+	return hipGetLastError();
+}
+
 hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int srcLayout,
-                                 const double* d_scale, void* d_dst, hipStream_t s)
+                                 const double* d_norm, void* d_dst, hipStream_t s)
 {
 	const uint64_t n = lay.nEvents * lay.rowLen;
 	const int bs = 256;
@@ -223,11 +252,11 @@ hipError_t launch_transform_amps(const AmpLayout& lay, const double* d_src, int 
 	if (lay.precision == RPWA_PREC_MIXED)
 		hipLaunchKernelGGL(transform_kernel<true>, dim3((uint32_t)nb), dim3(bs), 0, s,
 		                   reinterpret_cast<const double2*>(d_src), srcLayout, lay.nEvents, lay.nW[0],
-		                   lay.nW[1], lay.wmax, lay.secOff[1], lay.rowLen, d_scale, d_dst);
+		                   lay.nW[1], lay.wmax, lay.secOff[1], lay.rowLen, d_norm, d_dst);
 	else
 		hipLaunchKernelGGL(transform_kernel<false>, dim3((uint32_t)nb), dim3(bs), 0, s,
 		                   reinterpret_cast<const double2*>(d_src), srcLayout, lay.nEvents, lay.nW[0],
-		                   lay.nW[1], lay.wmax, lay.secOff[1], lay.rowLen, d_scale, d_dst);
+		                   lay.nW[1], lay.wmax, lay.secOff[1], lay.rowLen, d_norm, d_dst);
 	return hipGetLastError();
 }
 

@@ -633,16 +633,37 @@ bool pwaLikelihood<complexT>::addAmplitudesDevice(const double* d_amps, const un
 		          << "must hold the events of the bin only" << std::endl;
 		return false;
 	}
-	if (_useNormalizedAmps)
-		for (unsigned int r = 0; r < 2; ++r)
-			for (unsigned int w = 0; w < _nmbWavesRefl[r]; ++w)
-				if (_normMatrix[matIndex(r, w, r, w)] == 0.) {
-					// the reference rejects non-zero amplitudes of such a wave; on the
-					// device they are not inspected, so the wave is refused outright
-					std::cerr << "pwaLikelihood: normalization integral for wave '" << _waveNames[r][w]
-					          << "' is zero" << std::endl;
-					return false;
-				}
+	// the zero-amplitude rules of addAmplitude (pwaLikelihood.cc:1247-1281), with the
+	// columns inspected on the device only for waves with a zero integral diagonal
+	std::vector<std::string> toFix;
+	for (unsigned int r = 0; r < 2; ++r)
+		for (unsigned int w = 0; w < _nmbWavesRefl[r]; ++w) {
+			const bool accZero = _accMatrix[matIndex(r, w, r, w)] == 0.;
+			const bool normZero = _useNormalizedAmps and _normMatrix[matIndex(r, w, r, w)] == 0.;
+			if (not accZero and not normZero)
+				continue;
+			int nonzero = 0;
+			const uint32_t col = (r == 0 ? 0 : _nmbWavesRefl[0]) + w;
+			if (rpwa_wave_nonzero_device(_device, d_amps, nmbEvents, _nmbWaves, col, &nonzero) != RPWA_OK) {
+				std::cerr << "pwaLikelihood: inspecting the device amplitudes of wave '" << _waveNames[r][w]
+				          << "': " << rpwa_last_error() << std::endl;
+				return false;
+			}
+			if (normZero and nonzero) {
+				std::cerr << "pwaLikelihood: normalization integral for wave '" << _waveNames[r][w]
+				          << "' is zero, but the amplitude is not" << std::endl;
+				return false;
+			}
+			if (accZero and not nonzero)
+				toFix.push_back(_waveNames[r][w]);
+		}
+	for (const std::string& waveName : toFix)
+		for (fitParameter& par : _parameters)
+			if (par.waveName() == waveName and not par.fixed()) {
+				std::cerr << "pwaLikelihood: all amplitudes of wave '" << waveName
+				          << "' are zero, fixing its production amplitudes to zero" << std::endl;
+				par.fix(true);
+			}
 	_nmbEvents = nmbEvents;
 	_deviceAmps = d_amps;
 	for (unsigned int r = 0; r < 2; ++r)
@@ -665,14 +686,14 @@ bool pwaLikelihood<complexT>::finishInit()
 	for (unsigned int r = 0; r < 2; ++r)
 		for (unsigned int w = 0; w < _nmbWavesRefl[r]; ++w)
 			_phaseSpaceIntegral[r * _nmbWavesReflMax + w] = std::sqrt(_normMatrix[matIndex(r, w, r, w)].real());
-	// per-wave scale of device-resident raw amplitudes (1/sqrt(Re I_ii))
-	std::vector<double> waveScale(_nmbWaves, 1.);
+	// per-wave divisor of device-resident raw amplitudes (sqrt(Re I_ii), as addAmplitude divides)
+	std::vector<double> waveNorm(_nmbWaves, 1.);
 	if (_useNormalizedAmps) {
 		for (unsigned int r = 0; r < 2; ++r)
 			for (unsigned int w = 0; w < _nmbWavesRefl[r]; ++w) {
 				const std::complex<double> n = _normMatrix[matIndex(r, w, r, w)];
 				if (n != 0.)
-					waveScale[(r == 0 ? 0 : _nmbWavesRefl[0]) + w] = 1. / std::sqrt(n.real());
+					waveNorm[(r == 0 ? 0 : _nmbWavesRefl[0]) + w] = std::sqrt(n.real());
 			}
 		for (unsigned int iRefl = 0; iRefl < 2; ++iRefl)
 			for (unsigned int iWave = 0; iWave < _nmbWavesRefl[iRefl]; ++iWave) {
@@ -707,7 +728,7 @@ bool pwaLikelihood<complexT>::finishInit()
 		int rc;
 		if (_deviceAmps) {
 			rc = rpwa_lik_load_decay_amps_device(c, _deviceAmps, _nmbEvents, nW,
-			                                     _useNormalizedAmps ? waveScale.data() : nullptr);
+			                                     _useNormalizedAmps ? waveNorm.data() : nullptr);
 		} else {
 			// [N][W0 + W1], refl -1 waves first (already normalised on the host)
 			std::vector<std::complex<double> > rows((size_t)_nmbEvents * _nmbWaves);

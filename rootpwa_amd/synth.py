@@ -38,6 +38,24 @@ def generate_bin(seed, W, nData, mcFactor=5, device=0):
     return d_data, sums, M, truth[0, 1, :W].copy()
 
 
+def generate_sample(seed, nW, rank, nData, nMC, device=0):
+    """A single-bin workload on the device with both reflectivity sectors:
+    returns (device buffer [nData][W0+W1] complex128 of accept-rejected data,
+    integral sums [W][W] of an nMC-event MC pool, the true production
+    amplitudes [rank][2][Wmax], the size of the data pool)."""
+    W = nW[0] + nW[1]
+    scales = capi.gen_wave_scales(seed, W)
+    truth = capi.gen_prod_amps(seed, STREAM_TRUE_T, rank, nW)
+    d_mc = capi.DeviceBuffer(int(nMC) * W * 16, device)
+    capi.gen_amps_device(d_mc.address, seed, STREAM_MC, 0, int(nMC), scales, device)
+    sums = capi.integral_device(d_mc.address, int(nMC), W, device=device)
+    d_mc.free()
+    d_data = capi.DeviceBuffer(int(nData) * W * 16, device)
+    pool, _ = capi.gen_accept_reject_device(d_data.address, seed, STREAM_DATA, int(nData), nW, rank, truth, scales,
+                                            device)
+    return d_data, sums, truth, pool
+
+
 def fill_batch(batch, seed, sizes, mcFactor=5, device=0, log=None):
     """Adds one generated bin per entry of sizes to a rootpwa_amd.batch.BatchFit;
     returns the true production amplitudes [nBins][W]."""

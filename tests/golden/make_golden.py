@@ -17,9 +17,19 @@ Definitions (doc/tex/likelihood.tex:9-55, partialWaveFit/pwaLikelihood.cc):
   dL/dRe T, dL/dIm T: Re, Im of sum_e (-2/sigma_e) S_rs(e) conj(A_si(e))
                    + 2 nu sum_j T_rsj conj(I^acc_ij) - prior'(|T|)/|T| * T
   dL/df          = sum_e (-2 f / sigma_e) + 2 nu f eta
-with nu = 1 (normalised amplitudes), eta = 1 (acceptance = normalisation
-integral) and the parameter layout of buildParDataStruct (rank > sector (refl
--1 first) > wave; T_rsi = 0 for i < r, real for i = r; flat last).
+with the parameter layout of buildParDataStruct (rank > sector (refl -1
+first) > wave; T_rsi = 0 for i < r, real for i = r; flat last).  By default
+nu = 1 (normalised amplitudes) and the acceptance integral is the
+normalisation integral (eta = 1).  The acceptance cases integrate I^acc over an
+accepted subset of N_acc of the M MC events (addAccIntegral,
+pwaLikelihood.cc:1088-1107):
+  n_acc          = accEventsOverride if given, else M (the normalisation
+                   integral's event count)
+  eta            = N_acc / n_acc
+  element(I^acc) = sum over the accepted events / n_acc
+then rescaled by sqrt(I^norm_ii I^norm_jj) as above.  The non-normalised case
+(useNormalizedAmps(false)) keeps the raw amplitudes and I^acc, with
+nu = N (the number of data events, pwaLikelihood.cc:253).
 
 Inputs are drawn from a fixed numpy-free generator (Python's random.Random).
 Run: python tests/golden/make_golden.py
@@ -42,6 +52,17 @@ CASES = [
     ("w1+1_r1", (1, 1), 1, 20, 40, None, 4),
     ("w2+2_r2_cauchy", (2, 2), 2, 35, 70, 0.8, 5),
 ]
+# acceptance and normalisation variants: name -> (fraction of MC events
+# accepted, accEventsOverride, useNormalizedAmps)
+VARIANTS = {}
+CASES += [
+    ("w2+3_r2_acc", (2, 3), 2, 40, 80, None, 6),
+    ("w1+2_r1_acc_override_cauchy", (1, 2), 1, 30, 60, 0.5, 7),
+    ("w2+2_r2_unnormalised_acc", (2, 2), 2, 30, 64, None, 8),
+]
+VARIANTS["w2+3_r2_acc"] = (0.6, 0, True)
+VARIANTS["w1+2_r1_acc_override_cauchy"] = (0.45, 150, True)
+VARIANTS["w2+2_r2_unnormalised_acc"] = (0.5, 0, False)
 
 
 class C:
@@ -90,11 +111,14 @@ def draw_inputs(nW, rank, N, M, seed):
                 P += 2 * n - 1
     P += 1
     par = [rnd.uniform(-1.5, 1.5) for _ in range(P)]
-    return data, mc, par
+    return data, mc, par, rnd
 
 
-def golden(nW, rank, N, M, width, seed):
-    data, mc, par = draw_inputs(nW, rank, N, M, seed)
+def golden(nW, rank, N, M, width, seed, variant=None):
+    data, mc, par, rnd = draw_inputs(nW, rank, N, M, seed)
+    accFrac, override, normalised = variant if variant else (None, 0, True)
+    # accepted subset of the MC events (all of them without an acceptance variant)
+    accIdx = [k for k in range(M) if rnd.random() < accFrac] if accFrac is not None else list(range(M))
     W = nW[0] + nW[1]
     gw = lambda s, j: j if s == 0 else nW[0] + j
     # integral sums over the MC events (full W x W)
@@ -104,17 +128,27 @@ def golden(nW, rank, N, M, width, seed):
         for i in range(W):
             for j in range(W):
                 sums[i][j] = sums[i][j] + A[i] * A[j].conj()
+    accSums = [[C(0) for _ in range(W)] for _ in range(W)]
+    for k in accIdx:
+        A = [C(Decimal(x[0]), Decimal(x[1])) for x in mc[k]]
+        for i in range(W):
+            for j in range(W):
+                accSums[i][j] = accSums[i][j] + A[i] * A[j].conj()
     Md = Decimal(M)
     elem = [[C(sums[i][j].re / Md, sums[i][j].im / Md) for j in range(W)] for i in range(W)]
-    sq = [elem[i][i].re.sqrt() for i in range(W)]
-    # normalised data amplitudes and acceptance matrix (same-sector blocks)
+    nAcc = Decimal(override if override else M)
+    eta = Decimal(len(accIdx)) / nAcc
+    accElem = [[C(accSums[i][j].re / nAcc, accSums[i][j].im / nAcc) for j in range(W)] for i in range(W)]
+    sq = [elem[i][i].re.sqrt() if normalised else Decimal(1) for i in range(W)]
+    nu = Decimal(1) if normalised else Decimal(N)
+    # (normalised) data amplitudes and acceptance matrix (same-sector blocks)
     An = [[C(Decimal(ev[i][0]) / sq[i], Decimal(ev[i][1]) / sq[i]) for i in range(W)] for ev in data]
     Iacc = {}
     for s in range(2):
         for i in range(nW[s]):
             for j in range(nW[s]):
                 a, b = gw(s, i), gw(s, j)
-                Iacc[(s, i, j)] = C(elem[a][b].re / (sq[a] * sq[b]), elem[a][b].im / (sq[a] * sq[b]))
+                Iacc[(s, i, j)] = C(accElem[a][b].re / (sq[a] * sq[b]), accElem[a][b].im / (sq[a] * sq[b]))
     # parameters -> production amplitudes
     T = {}
     idx = {}
@@ -152,8 +186,8 @@ def golden(nW, rank, N, M, width, seed):
         for (r, s, i) in T:
             G[(r, s, i)] = G[(r, s, i)] + S[(r, s)] * A[gw(s, i)].conj() * w
         gflat += w * f
-    # normalisation term (nu = 1, eta = 1)
-    norm = f * f
+    # normalisation term: nu (sum Re(T_i T_j* I^acc_ij) + f^2 eta)
+    norm = f * f * eta
     for r in range(rank):
         for s in range(2):
             for i in range(nW[s]):
@@ -162,9 +196,9 @@ def golden(nW, rank, N, M, width, seed):
                     I = Iacc[(s, i, j)]
                     norm += ((T[(r, s, i)] * T[(r, s, j)].conj()) * I).re
                     dn = dn + T[(r, s, j)] * I.conj()
-                G[(r, s, i)] = G[(r, s, i)] + dn * Decimal(2)
-    gflat += Decimal(2) * f
-    value = lnl + norm
+                G[(r, s, i)] = G[(r, s, i)] + dn * (Decimal(2) * nu)
+    gflat += Decimal(2) * nu * f * eta
+    value = lnl + nu * norm
     # half-Cauchy prior
     if width is not None:
         g2 = Decimal(width) * Decimal(width)
@@ -181,10 +215,16 @@ def golden(nW, rank, N, M, width, seed):
         if ki is not None:
             grad[ki] = float(G[key].im)
     grad[-1] = float(gflat)
-    return dict(nW=list(nW), rank=rank, events=N, mc_events=M, cauchy_width=width,
-                data=data, mc=mc, par=par,
-                integral_sums=[[[float(sums[i][j].re), float(sums[i][j].im)] for j in range(W)] for i in range(W)],
-                value=float(value), event_part=float(lnl), norm_term=float(norm), gradient=grad)
+    out = dict(nW=list(nW), rank=rank, events=N, mc_events=M, cauchy_width=width,
+               data=data, mc=mc, par=par,
+               integral_sums=[[[float(sums[i][j].re), float(sums[i][j].im)] for j in range(W)] for i in range(W)],
+               value=float(value), event_part=float(lnl), norm_term=float(norm), gradient=grad)
+    if variant:
+        out.update(acc_index=accIdx, acc_events=len(accIdx), acc_override=override, normalised=normalised,
+                   eta=float(eta),
+                   acc_sums=[[[float(accSums[i][j].re), float(accSums[i][j].im)] for j in range(W)]
+                             for i in range(W)])
+    return out
 
 
 def main():
@@ -192,7 +232,7 @@ def main():
                           "tests/golden/make_golden.py in 60-digit decimal arithmetic",
            "cases": {}}
     for name, nW, rank, N, M, width, seed in CASES:
-        out["cases"][name] = golden(nW, rank, N, M, width, seed)
+        out["cases"][name] = golden(nW, rank, N, M, width, seed, VARIANTS.get(name))
         print(name, out["cases"][name]["value"])
     with open(OUT, "w") as fh:
         json.dump(out, fh, indent=0)

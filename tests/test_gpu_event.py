@@ -111,7 +111,10 @@ def test_reshuffled_layout(gpu):
 
 def test_integral_matches_oracle(gpu):
     rng = np.random.default_rng(2)
-    for W, M in ((3, 1000), (16, 5000), (88, 3001), (110, 700)):
+    # 57-64, 121-128 and 249-256 waves: the widest chunk buffers of each row padding
+    for W, M in ((3, 1000), (16, 5000), (57, 900), (64, 1200), (88, 3001), (110, 700), (121, 300), (128, 400),
+                 (256, 200)):
+        print("integral W", W, "M", M, flush=True)
         a = rng.normal(size=(M, W)) + 1j * rng.normal(size=(M, W))
         ref = O.integral(a)
         got = gpu.integral(a)

@@ -40,9 +40,17 @@ def test_golden_vectors_on_gpu(gpu, name, precision):
     I.integrate(_cplx(c["mc"]))
     sums, n = I.sums()
     assert n == c["mc_events"] and H.rel_err(sums, _cplx(c["integral_sums"])) <= 1e-13
+    Iacc = None
+    if "acc_sums" in c:
+        # acceptance integral over the accepted MC subset, also accumulated on the GPU
+        Iacc = PL.ampIntegralMatrix(names)
+        Iacc.integrate(_cplx(c["mc"])[np.array(c["acc_index"], dtype=np.int64)])
+        sums, n = Iacc.sums()
+        assert n == c["acc_events"] and H.rel_err(sums, _cplx(c["acc_sums"])) <= 1e-13
     data = _cplx(c["data"])
-    L = PL.initLikelihood(waves, {names[k]: data[:, k] for k in range(len(names))}, I, rank=c["rank"],
-                          precision=precision, cauchyWidth=c["cauchy_width"])
+    L = PL.initLikelihood(waves, {names[k]: data[:, k] for k in range(len(names))}, I, accIntegral=Iacc,
+                          accEventsOverride=c.get("acc_override", 0), rank=c["rank"], precision=precision,
+                          cauchyWidth=c["cauchy_width"], useNormalizedAmps=c.get("normalised", True))
     par = np.array(c["par"])
     f, g = L.FdF(par)
     gref = np.array(c["gradient"])

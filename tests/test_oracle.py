@@ -33,7 +33,9 @@ def oracle_from_golden(c):
     L = O.OracleLikelihood(tuple(c["nW"]), c["rank"])
     if c["cauchy_width"] is not None:
         L.set_prior(True, c["cauchy_width"])
-    L.add_integrals(_cplx(c["integral_sums"]), c["mc_events"])
+    L.use_normalized_amps(c.get("normalised", True))
+    acc = _cplx(c["acc_sums"]) if "acc_sums" in c else None
+    L.add_integrals(_cplx(c["integral_sums"]), c["mc_events"], acc, c.get("acc_events"), c.get("acc_override", 0))
     L.add_amplitudes(_cplx(c["data"]))
     L.finish_init()
     return L
@@ -51,6 +53,52 @@ def test_golden_vectors(name):
     assert L.doeval(par) == f
     assert np.array_equal(L.gradient(par), g)
     assert H.rel_err(O.integral(_cplx(c["mc"])), _cplx(c["integral_sums"])) <= 1e-14
+    if "acc_sums" in c:
+        # the acceptance integral is the integral over the accepted MC subset
+        acc = _cplx(c["mc"])[np.array(c["acc_index"], dtype=np.int64)]
+        assert H.rel_err(O.integral(acc), _cplx(c["acc_sums"])) <= 1e-14
+        assert L.tot_acc() == c["acc_events"] / (c["acc_override"] or c["mc_events"])
+
+
+@pytest.mark.parametrize("name", list(golden_cases().keys()))
+@pytest.mark.parametrize("threads", [2, 3, 7, 16, 64])
+def test_threaded_fdf_on_golden_vectors(name, threads):
+    """orc_lik_fdf_mt (the checker of the full-size GPU runs) reproduces the
+    golden vectors, also with more threads than events (empty ranges)."""
+    c = golden_cases()[name]
+    L = oracle_from_golden(c)
+    par = np.array(c["par"])
+    f, g = L.fdf(par, threads=threads)
+    assert abs(f - c["value"]) <= 1e-13 * max(1.0, abs(c["value"]))
+    gref = np.array(c["gradient"])
+    assert np.max(np.abs(g - gref)) <= 1e-13 * np.max(np.abs(gref))
+
+
+@pytest.mark.parametrize("threads", list(range(1, 17)))
+def test_threaded_fdf_equals_single_thread(threads):
+    """Range partials merged with their compensations: orc_lik_fdf_mt equals
+    orc_lik_fdf to the rounding of the Kahan merge (a few ulp of the sums)."""
+    ds = H.dataset((3, 9), 2, 3001, 2000, seed=77)
+    L = H.oracle_likelihood(ds)
+    rng = np.random.default_rng(threads)
+    par = H.random_par(rng, L, 0.7)
+    f1, g1 = L.fdf(par)
+    fm, gm = L.fdf(par, threads=threads)
+    assert abs(fm - f1) <= 4e-16 * 3001 ** 0.5 * abs(f1) + 1e-12
+    assert np.max(np.abs(gm - g1)) <= 1e-14 * np.max(np.abs(g1))
+    if threads == 1:
+        assert fm == f1 and np.array_equal(gm, g1)
+
+
+def test_threaded_fdf_fewer_events_than_threads():
+    ds = H.dataset((2, 3), 1, 5, 200, seed=78)
+    L = H.oracle_likelihood(ds)
+    par = H.random_par(np.random.default_rng(5), L, 0.7)
+    f1, g1 = L.fdf(par)
+    for threads in (6, 16):
+        fm, gm = L.fdf(par, threads=threads)
+        assert abs(fm - f1) <= 1e-15 * abs(f1) + 1e-14
+        assert np.max(np.abs(gm - g1)) <= 1e-15 * np.max(np.abs(g1))
 
 
 def test_kahan_scenario_of_the_reference():
@@ -115,6 +163,37 @@ def test_closed_form_one_wave():
     L2.finish_init()
     expect2 = -np.sum(np.log(T * T * np.abs(a) ** 2 + f * f)) + 50 * (T * T * Isum / 400 + f * f)
     assert L2.doeval(np.array([T, f])) == pytest.approx(expect2, rel=1e-14)
+
+
+@pytest.mark.parametrize("override", [0, 1000])
+def test_closed_form_one_wave_acceptance(override):
+    """One wave, rank 1, acceptance integral over an accepted subset
+    (addAccIntegral, pwaLikelihood.cc:1088-1107): with n = accEventsOverride
+    or the MC count, eta = N_acc / n and
+    -lnL = -sum ln(T^2 |A|^2 / I_n + f^2) + T^2 (S_acc / n) / I_n + f^2 eta,
+    I_n = S_mc / M; computed here from the raw sums."""
+    rng = np.random.default_rng(11)
+    a = rng.normal(size=40) + 1j * rng.normal(size=40)
+    mc = rng.normal(size=600) + 1j * rng.normal(size=600)
+    accepted = rng.random(600) < 0.3
+    S_mc = np.sum(np.abs(mc) ** 2)
+    S_acc = np.sum(np.abs(mc[accepted]) ** 2)
+    n_acc = int(accepted.sum())
+    L = O.OracleLikelihood((0, 1), 1)
+    L.add_integrals(np.array([[S_mc]]), 600, np.array([[S_acc]]), n_acc, override)
+    L.add_amplitudes(a[:, None])
+    L.finish_init()
+    T, f = 1.3, 0.4
+    n = override or 600
+    I_n = S_mc / 600
+    eta = n_acc / n
+    expect = -np.sum(np.log(T * T * np.abs(a) ** 2 / I_n + f * f)) + T * T * (S_acc / n) / I_n + f * f * eta
+    assert L.doeval(np.array([T, f])) == pytest.approx(expect, rel=1e-14)
+    # gradient: d/dT = sum -2 T |A|^2/I_n / sigma + 2 T (S_acc/n)/I_n, d/df = sum -2 f / sigma + 2 f eta
+    sigma = T * T * np.abs(a) ** 2 / I_n + f * f
+    g = L.gradient(np.array([T, f]))
+    assert g[0] == pytest.approx(np.sum(-2 * T * np.abs(a) ** 2 / I_n / sigma) + 2 * T * S_acc / n / I_n, rel=1e-13)
+    assert g[1] == pytest.approx(np.sum(-2 * f / sigma) + 2 * f * eta, rel=1e-13)
 
 
 def numpy_neg_lnl(L, prod, flat):
