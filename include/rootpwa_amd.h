@@ -203,6 +203,18 @@ int rpwa_integral(int device, const double* amps, uint64_t nmbEvents, uint32_t n
 /* amplitudes (and weights) already in device memory */
 int rpwa_integral_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
                          const double* d_weights, double* outSums);
+/* rpwa_integral_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bitwise equal to rpwa_integral_device on a
+ * GPU with at most 256 compute units); the work area is one static device
+ * array of 68.5 MiB that each GPU the library runs on holds until the process
+ * ends.  At most 128 waves (RPWA_ERR_UNSUPPORTED above).  Calls share that
+ * array per GPU: calls that overlap on one GPU (from several host threads)
+ * race.  Runs on the device's null stream and returns after the result is in
+ * outSums. */
+int rpwa_integral_device_noalloc(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
+                                 const double* d_weights, double* outSums);
 /* Data moments for the learned initializer (no counterpart in the reference):
  * with a_i(e) = A_i(e) / sqrt(normDiag_i) (normDiag: the normalisation
  * integral's diagonal elements), outMoments [W][W] complex =

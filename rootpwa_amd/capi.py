@@ -68,6 +68,7 @@ PROTOTYPES = {
     "rpwa_bandwidth_probe": (_i, [_i, _u64, _i, _dp, _dp]),
     "rpwa_integral": (_i, [_i, _dp, _u64, _u32, _dp, _dp]),
     "rpwa_integral_device": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
+    "rpwa_integral_device_noalloc": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
     "rpwa_data_moments_device": (_i, [_i, _vp, _u64, _u32, _dp, _i, _dp]),
     "rpwa_gen_wave_scales": (_i, [_u64, _u32, _dp]),
     "rpwa_gen_prod_amps": (_i, [_u64, _u32, _u32, _u32p, _dp]),
@@ -327,6 +328,14 @@ def integral(amps, weights=None, device=0):
 def integral_device(d_amps, M, W, d_weights=None, device=0):
     out = np.zeros((W, W), dtype=np.complex128)
     call("rpwa_integral_device", int(device), _vp(d_amps), int(M), int(W),
+         _vp(d_weights) if d_weights else None, _cd(out))
+    return out
+
+
+def integral_device_noalloc(d_amps, M, W, d_weights=None, device=0):
+    """rpwa_integral_device_noalloc: the same sums from a static device work area."""
+    out = np.zeros((W, W), dtype=np.complex128)
+    call("rpwa_integral_device_noalloc", int(device), _vp(d_amps), int(M), int(W),
          _vp(d_weights) if d_weights else None, _cd(out))
     return out
 

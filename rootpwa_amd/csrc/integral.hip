@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -499,7 +500,153 @@ static int moments_run(int device, const double* d_amps, uint64_t M, uint32_t W,
 	return RPWA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The integral without allocations (rpwa_integral_device_noalloc): the same
+// kernels and reduction order as integral_run on device input, with every
+// intermediate in one static device array instead of hipMalloc'd buffers.
+// Layout of the array (doubles): workgroup partial tiles
+// [kNaWg][kNaTiles][256], workgroup weight sums [kNaWg], reduced tiles
+// [kNaTiles][256], the complex result [W][W].  Every element a call reads is
+// written earlier in the same call, so nothing needs clearing between calls.
+constexpr uint32_t kNaMaxWaves = 128;                       // D = 256: 16 x 16 tiles of 16 x 16
+constexpr uint32_t kNaTiles = (2 * kNaMaxWaves / 16) * (2 * kNaMaxWaves / 16 + 1) / 2;  // 136
+constexpr int kNaWg = 256;                                  // one workgroup per CU of an MI355X
+constexpr size_t kNaPartial = (size_t)kNaWg * kNaTiles * 256;
+constexpr size_t kNaWsum = kNaPartial;
+constexpr size_t kNaRed = kNaWsum + kNaWg;
+constexpr size_t kNaOut = kNaRed + (size_t)kNaTiles * 256;
+constexpr size_t kNaDoubles = kNaOut + 2ull * kNaMaxWaves * kNaMaxWaves;  // 8,980,736 doubles = 68.5 MiB
+constexpr int kNaMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+namespace {
+
+// per-device address of `scratch`, looked up once per device with that device current
+std::once_flag g_naOnce[kNaMaxDevices];
+double* g_naPtr[kNaMaxDevices];
+hipError_t g_naErr[kNaMaxDevices];
+
+// the complex sums [W][W] from the reduced tiles (the host assembly of integral_run),
+// renormalised by the mean weight when weights are given (ampIntegralMatrix.cc:477-483)
+__global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict__ red, uint32_t nT, uint32_t W,
+                                                       const double* __restrict__ wsum, int nWg, uint64_t M,
+                                                       int weighted, double* __restrict__ out)
+{
+	const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= W * W)
+		return;
+	const uint32_t i = idx / W, j = idx % W;
+	auto c = [&](uint32_t r, uint32_t col) {
+		uint32_t I = r / 16, J = col / 16;
+		if (I > J) {
+			const uint32_t t = r;
+			r = col;
+			col = t;
+			const uint32_t u = I;
+			I = J;
+			J = u;
+		}
+		const uint32_t t = I * nT - I * (I - 1) / 2 + (J - I);
+		return red[(size_t)t * 256 + (r % 16) * 16 + (col % 16)];
+	};
+	double re = c(2 * i, 2 * j) + c(2 * i + 1, 2 * j + 1);
+	double im = c(2 * i + 1, 2 * j) - c(2 * i, 2 * j + 1);
+	if (weighted) {
+		double t = 0.;
+		for (int r = 0; r < nWg; ++r)
+			t += wsum[r];
+		const double weightNorm = t / (double)M;
+		const double inv = 1 / weightNorm;
+		re *= inv;
+		im *= inv;
+	}
+	out[2 * (size_t)idx] = re;
+	out[2 * (size_t)idx + 1] = im;
+}
+
+}  // namespace
+
+static int integral_noalloc(int device, const double* d_amps, uint64_t M, uint32_t W, const double* d_w,
+                            double* outSums)
+{
+	if (W > kNaMaxWaves) {
+		set_error("rpwa_integral_device_noalloc: " + std::to_string(W) + " waves exceed the static work area (at "
+		          "most " + std::to_string(kNaMaxWaves) + "); use rpwa_integral_device");
+		return RPWA_ERR_UNSUPPORTED;
+	}
+	int nDev = 0;
+	RPWA_HIP_CHECK(hipGetDeviceCount(&nDev));
+	if (device < 0 || device >= nDev || device >= kNaMaxDevices) {
+		set_error("rpwa_integral_device_noalloc: no GPU with index " + std::to_string(device));
+		return RPWA_ERR_INVALID;
+	}
+	RPWA_HIP_CHECK(hipSetDevice(device));
+	std::call_once(g_naOnce[device], [device]() {
+		g_naErr[device] = hipGetSymbolAddress(reinterpret_cast<void**>(&g_naPtr[device]), HIP_SYMBOL(scratch));
+	});
+	if (g_naErr[device] != hipSuccess || !g_naPtr[device]) {
+		set_error(std::string("rpwa_integral_device_noalloc: hipGetSymbolAddress(scratch): ") +
+		          hipGetErrorString(g_naErr[device]));
+		return RPWA_ERR_HIP;
+	}
+	double* const base = g_naPtr[device];
+	IntegralPlan p;
+	if (int st = integral_plan(device, 2 * W, p))
+		return st;
+	p.nWg = std::min(p.nWg, kNaWg);
+	const bool weighted = d_w != nullptr;
+	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
+	if (p.lds + 32 * sizeof(double) > 64u * 1024u)
+		RPWA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
+	IntArgs a;
+	a.x = d_amps;
+	a.w = d_w;
+	a.M = M;
+	a.D = p.D;
+	a.Dp = p.Dp;
+	a.ldsStride = p.ldsStride;
+	a.nT = p.nT;
+	a.nTiles = p.nTiles;
+	a.partials = base;
+	a.wsum = base + kNaWsum;
+	a.accumulate = 0;
+	a.directWeights = 0;
+	for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
+		a.tileBase = tb;
+		a.tileEnd = std::min(p.nTiles, tb + p.tilesPerPass);
+		hipLaunchKernelGGL(k, dim3(p.nWg), dim3(kIntThreads), p.lds, 0, a);
+		RPWA_HIP_CHECK(hipGetLastError());
+	}
+	const int nb = (int)((p.nTiles * 256 + kColElems - 1) / kColElems);
+	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, 0, base, p.nWg,
+	                   p.nTiles * 256, base + kNaRed);
+	RPWA_HIP_CHECK(hipGetLastError());
+	hipLaunchKernelGGL(assemble_kernel, dim3((W * W + 255) / 256), dim3(256), 0, 0, base + kNaRed, p.nT, W,
+	                   base + kNaWsum, p.nWg, M, weighted ? 1 : 0, base + kNaOut);
+	RPWA_HIP_CHECK(hipGetLastError());
+	RPWA_HIP_CHECK(hipMemcpy(outSums, base + kNaOut, (size_t)W * W * 2 * sizeof(double), hipMemcpyDeviceToHost));
+	return RPWA_OK;
+}
+
 }  // namespace rpwa_amd
+
+extern "C" int rpwa_integral_device_noalloc(int device, const double* d_amps, uint64_t nmbEvents,
+                                            uint32_t nmbWaves, const double* d_weights, double* outSums)
+{
+	rpwa_amd::clear_error();
+	if (!d_amps || !outSums || nmbEvents == 0 || nmbWaves == 0) {
+		rpwa_amd::set_error("rpwa_integral_device_noalloc: null pointer or empty input");
+		return RPWA_ERR_INVALID;
+	}
+	try {
+		return rpwa_amd::integral_noalloc(device, d_amps, nmbEvents, nmbWaves, d_weights, outSums);
+	} catch (const std::exception& ex) {
+		rpwa_amd::set_error(ex.what());
+		return RPWA_ERR_INVALID;
+	}
+}
 
 extern "C" int rpwa_data_moments_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
                                         const double* normDiag, int perEventNorm, double* outMoments)
