@@ -224,6 +224,26 @@ int rpwa_data_moments_device(int device, const double* d_amps, uint64_t nmbEvent
                              const double* normDiag, int perEventNorm, double* outMoments);
 
 /* ---------------------------------------------------------------------------
+ * Model intensity of generated events: modelIntensity::getIntensity
+ * (generators/modelIntensity.cc:170-187), with the decay amplitudes of
+ * getDecayAmplitudes (:190-211) given as arrays,
+ *   intensity[e] = sum_r sum_refl | sum_i T_r,refl,i A_refl,i(e) / waveNorm_refl,i |^2
+ * amps: [nmbEvents][W0 + W1] complex128, refl -1 waves first; prodAmps (host):
+ * [rank][2][nmbWavesMax] complex, nmbProdAmps = rank*2*nmbWavesMax (rank 1 with
+ * the fit result's production amplitudes is the reference's quantity; the flat
+ * wave is not included); waveNorm (host, [W0 + W1], may be NULL): the phase-space
+ * integrals sqrt(Re I_ii) the amplitudes are divided by.  FP64.
+ * --------------------------------------------------------------------------- */
+/* amplitudes and result in device memory (d_intensity [nmbEvents] doubles) */
+int rpwa_intensity_device(int device, const double* d_amps, uint64_t nmbEvents, const uint32_t nmbWavesRefl[2],
+                          const double* prodAmps, uint32_t nmbProdAmps, uint32_t rank, const double* waveNorm,
+                          double* d_intensity);
+/* amplitudes and result in host memory (copied through the device) */
+int rpwa_intensity(int device, const double* amps, uint64_t nmbEvents, const uint32_t nmbWavesRefl[2],
+                   const double* prodAmps, uint32_t nmbProdAmps, uint32_t rank, const double* waveNorm,
+                   double* intensity);
+
+/* ---------------------------------------------------------------------------
  * Synthetic amplitudes (the generator behind the benchmark workloads).
  * A_i(e) = (N(0,1) + i N(0,1)) * s_i * exp(0.5 N(0,1)_e), drawn from a
  * counter-based hash of (seed, stream, event, slot), so any event of a pool

@@ -75,6 +75,7 @@ def lib():
         L.orc_event_deriv.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, _dp]
         L.orc_integral.argtypes = [_dp, ctypes.c_uint64, ctypes.c_uint, _dp, _dp]
         L.orc_integral_incoherent.argtypes = [_dp, ctypes.c_uint, ctypes.c_uint64, ctypes.c_uint, _dp, _dp]
+        L.orc_intensity.argtypes = [_dp, ctypes.c_uint64, _up, _dp, ctypes.c_uint, _dp, _dp]
         L.orc_kahan_float_demo.argtypes = [ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]
         L.orc_gen_hash.restype = ctypes.c_uint64
@@ -276,6 +277,18 @@ def integral(amps, weights=None):
     M, W = a.shape
     out = np.zeros((W, W), dtype=np.complex128)
     lib().orc_integral(a.view(np.float64).ctypes.data_as(_dp), M, W, wp, out.view(np.float64).ctypes.data_as(_dp))
+    return out
+
+
+def intensity(amps, nW, prod, waveNorm=None):
+    """modelIntensity::getIntensity per event of amps [N][W0+W1], prod [rank][2][Wmax]."""
+    a = cplx_array(amps)
+    p = cplx_array(prod)
+    out = np.zeros(a.shape[0])
+    nw = (ctypes.c_uint * 2)(*nW)
+    n = None if waveNorm is None else np.ascontiguousarray(waveNorm, dtype=np.float64)
+    lib().orc_intensity(a.view(np.float64).ctypes.data_as(_dp), a.shape[0], nw, p.view(np.float64).ctypes.data_as(_dp),
+                        p.shape[0], _d(n) if n is not None else None, _d(out))
     return out
 
 

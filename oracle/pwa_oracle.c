@@ -898,6 +898,37 @@ void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weig
 	orc_integral_incoherent(amps, 1, M, W, weights, out);
 }
 
+/* modelIntensity::getIntensity (generators/modelIntensity.cc:170-187); the
+ * division of each decay amplitude by its phase-space integral is
+ * getDecayAmplitudes' (:190-211), the loop over reflectivities the set
+ * _allRefls in ascending order (-1, +1), extended by an inner loop over ranks */
+void orc_intensity(const double* amps, uint64_t N, const unsigned nWavesRefl[2], const double* prodAmps,
+                   unsigned rank, const double* waveNorm, double* out)
+{
+	const unsigned W = nWavesRefl[0] + nWavesRefl[1];
+	const unsigned wmax = nWavesRefl[0] > nWavesRefl[1] ? nWavesRefl[0] : nWavesRefl[1];
+	for (uint64_t e = 0; e < N; ++e) {
+		double intensity = 0;
+		for (unsigned refl = 0; refl < 2; ++refl) {
+			if (nWavesRefl[refl] == 0)
+				continue;  /* a reflectivity without waves is not in _allRefls */
+			for (unsigned r = 0; r < rank; ++r) {
+				cplx amp = c_make(0, 0);
+				for (unsigned i = 0; i < nWavesRefl[refl]; ++i) {
+					const unsigned col = (refl == 0 ? 0 : nWavesRefl[0]) + i;
+					cplx a = c_ld(amps + 2 * ((size_t)e * W + col));
+					if (waveNorm)
+						a = c_make(a.re / waveNorm[col], a.im / waveNorm[col]);
+					const cplx t = c_ld(prodAmps + 2 * (((size_t)r * 2 + refl) * wmax + i));
+					amp = c_add(amp, c_mul(t, a));
+				}
+				intensity += c_norm(amp);
+			}
+		}
+		out[e] = intensity;
+	}
+}
+
 /* testSumAccumulators (test/utilities/testSumAccumulators.cc:46-60) */
 void orc_kahan_float_demo(uint64_t n, float x, float* naive, float* compensated)
 {

@@ -116,6 +116,16 @@ void orc_integral_incoherent(const double* amps, unsigned S, uint64_t M, unsigne
 void orc_integral(const double* amps, uint64_t M, unsigned W, const double* weights,
                   double* out);
 
+/* modelIntensity::getIntensity (generators/modelIntensity.cc:170-187) with the
+ * decay amplitudes of getDecayAmplitudes (:190-211) given per event:
+ * out[e] = sum over reflectivities (-1, then +1), then ranks, of
+ * |sum_i T_r,refl,i (A_refl,i(e) / waveNorm_i)|^2 (plain std::complex
+ * accumulation, as the reference's `amp +=`).  amps [N][W0+W1] complex (refl -1
+ * first), prodAmps [rank][2][Wmax] complex, waveNorm [W0+W1] or NULL (no
+ * division).  Rank 1 is the reference's quantity. */
+void orc_intensity(const double* amps, uint64_t N, const unsigned nWavesRefl[2], const double* prodAmps,
+                   unsigned rank, const double* waveNorm, double* out);
+
 /* the testSumAccumulators scenario (test/utilities/testSumAccumulators.cc:46-60):
  * n additions of x in float, returned as {naive, compensated}. */
 void orc_kahan_float_demo(uint64_t n, float x, float* naive, float* compensated);

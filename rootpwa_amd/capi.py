@@ -70,6 +70,8 @@ PROTOTYPES = {
     "rpwa_integral_device": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
     "rpwa_integral_device_noalloc": (_i, [_i, _vp, _u64, _u32, _vp, _dp]),
     "rpwa_data_moments_device": (_i, [_i, _vp, _u64, _u32, _dp, _i, _dp]),
+    "rpwa_intensity_device": (_i, [_i, _vp, _u64, _u32p, _dp, _u32, _u32, _dp, _vp]),
+    "rpwa_intensity": (_i, [_i, _dp, _u64, _u32p, _dp, _u32, _u32, _dp, _dp]),
     "rpwa_gen_wave_scales": (_i, [_u64, _u32, _dp]),
     "rpwa_gen_prod_amps": (_i, [_u64, _u32, _u32, _u32p, _dp]),
     "rpwa_gen_amps_device": (_i, [_i, _u64, _u32, _u64, _u64, _u32, _dp, _vp]),
@@ -348,6 +350,29 @@ def data_moments_device(d_amps, N, W, normDiag, perEventNorm=True, device=0):
     call("rpwa_data_moments_device", int(device), _vp(d_amps), int(N), int(W), _dptr(nd), int(bool(perEventNorm)),
          _cd(out))
     return out
+
+
+def intensity(amps, nW, prod, waveNorm=None, device=0):
+    """Per-event model intensity sum_r sum_refl |sum_i T a_i|^2 of host amplitudes
+    [N][W0+W1] and production amplitudes [rank][2][Wmax] (rpwa_intensity)."""
+    a = _cplx(amps)
+    p = _cplx(prod)
+    rank = p.shape[0]
+    nw = (ctypes.c_uint32 * 2)(*nW)
+    out = np.zeros(a.shape[0])
+    nrm = None if waveNorm is None else np.ascontiguousarray(waveNorm, dtype=np.float64)
+    call("rpwa_intensity", int(device), _cd(a), a.shape[0], nw, _cd(p), p.size, rank,
+         _dptr(nrm) if nrm is not None else None, _dptr(out))
+    return out
+
+
+def intensity_device(d_amps, N, nW, prod, d_out, waveNorm=None, device=0):
+    """rpwa_intensity_device: amplitudes [N][W0+W1] and the result [N] in device memory."""
+    p = _cplx(prod)
+    nw = (ctypes.c_uint32 * 2)(*nW)
+    nrm = None if waveNorm is None else np.ascontiguousarray(waveNorm, dtype=np.float64)
+    call("rpwa_intensity_device", int(device), _vp(d_amps), int(N), nw, _cd(p), p.size, p.shape[0],
+         _dptr(nrm) if nrm is not None else None, _vp(d_out))
 
 
 def gen_wave_scales(seed, W):

@@ -196,6 +196,47 @@ def test_closed_form_one_wave_acceptance(override):
     assert g[1] == pytest.approx(np.sum(-2 * f / sigma) + 2 * f * eta, rel=1e-13)
 
 
+def test_intensity_closed_form_and_brute_force():
+    """orc_intensity (modelIntensity::getIntensity): one wave, rank 1:
+    |T|^2 |A|^2 / n^2; several waves and ranks: an independent einsum."""
+    rng = np.random.default_rng(12)
+    a = rng.normal(size=(30, 1)) + 1j * rng.normal(size=(30, 1))
+    T = np.array([[[0.0], [0.8 - 0.3j]]])
+    got = O.intensity(a, (0, 1), T, waveNorm=np.array([1.7]))
+    assert np.allclose(got, abs(0.8 - 0.3j) ** 2 * np.abs(a[:, 0]) ** 2 / 1.7 ** 2, rtol=1e-14, atol=0)
+    nW, rank = (3, 5), 3
+    a = rng.normal(size=(50, 8)) + 1j * rng.normal(size=(50, 8))
+    prod = H.random_prod(rng, rank, nW)
+    n = rng.uniform(0.5, 2.0, size=8)
+    an = a / n
+    expect = np.zeros(50)
+    for s, cols in ((0, slice(0, 3)), (1, slice(3, 8))):
+        S = np.einsum("ri,ei->er", prod[:, s, :nW[s]], an[:, cols])
+        expect += np.sum(np.abs(S) ** 2, axis=1)
+    assert np.allclose(O.intensity(a, nW, prod, n), expect, rtol=1e-13, atol=0)
+    # ranks add up; a global phase per (rank, reflectivity) block changes nothing
+    per_rank = sum(O.intensity(a, nW, prod[r:r + 1], n) for r in range(rank))
+    assert np.allclose(O.intensity(a, nW, prod, n), per_rank, rtol=1e-14, atol=0)
+    rot = prod.copy()
+    rot[1, 0] *= np.exp(0.7j)
+    rot[2, 1] *= np.exp(-2.1j)
+    assert np.allclose(O.intensity(a, nW, rot, n), expect, rtol=1e-13, atol=0)
+
+
+def test_intensity_is_the_likelihood_event_term(small):
+    """With the likelihood's normalisation (raw amplitudes divided by
+    sqrt(Re I_ii / M)), sum_e -ln(intensity_e + f^2) is the golden-pinned
+    likelihood oracle's event part."""
+    ds, L = small
+    prod = H.random_prod(np.random.default_rng(4), 2, ds["nW"], 0.7)
+    norm = np.sqrt(np.real(np.diag(ds["integral"])) / ds["nMC"])
+    inten = O.intensity(ds["data"], ds["nW"], prod, norm)
+    assert -np.sum(np.log(inten + 0.3 ** 2)) == pytest.approx(L.event_loglik(prod, 0.3), rel=1e-13)
+    # the same from the likelihood's own normalised amplitudes without a divisor
+    inten2 = O.intensity(H.normalised_amps(L), ds["nW"], prod)
+    assert np.allclose(inten2, inten, rtol=1e-13, atol=0)
+
+
 def numpy_neg_lnl(L, prod, flat):
     """Independent vectorised formulation (einsum over the sector blocks)."""
     A = [L.decay_amps(0), L.decay_amps(1)]
