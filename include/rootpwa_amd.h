@@ -82,7 +82,9 @@ int rpwa_lik_load_decay_amps_device(rpwa_lik_ctx* ctx, const double* d_amps, uin
 
 /* Event part of -lnL: sum_e -ln( sum_r sum_refl |sum_i T_r,refl,i A_refl,i(e)|^2 + flat^2 )
  * (replaces likelihoodInterface::logLikelihood, likelihoodInterface.cuh:112-115).
- * prodAmps: [rank][2][nmbWavesMax] complex, nmbProdAmps = rank*2*nmbWavesMax. */
+ * prodAmps: [rank][2][nmbWavesMax] complex, nmbProdAmps = rank*2*nmbWavesMax.
+ * Any rank: up to 4 in one fused pass, higher ranks in blocks of 4 through the
+ * LDS-staged kernel (2 x blocks passes over the amplitudes). */
 int rpwa_lik_log_likelihood(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
                             double prodAmpFlat, uint32_t rank, double* logLikelihood);
 

@@ -129,9 +129,13 @@ void free_buffers(rpwa_lik_ctx* c)
 	hipFree(c->d_amps);
 	hipFree(c->d_prod);
 	hipFree(c->d_partials);
+	hipFree(c->d_sigma);
 	hipHostFree(c->h_prod);
 	hipHostFree(c->h_out);
 	c->d_amps = nullptr;
+	c->d_sigma = nullptr;
+	c->rankCap = 0;
+	c->partialCap = 0;
 	c->d_prod = nullptr;
 	c->d_partials = nullptr;
 	c->d_out = nullptr;
@@ -140,25 +144,55 @@ void free_buffers(rpwa_lik_ctx* c)
 	c->loaded = false;
 }
 
-// (Re)compute the persistent grid of a rank's tile.
-int configure_grid(rpwa_lik_ctx* c, int rank)
+// (Re)compute the persistent grid of a rank slot's tile (the high-rank slot
+// runs the staged kernel instantiated for kMaxRank ranks).
+int configure_grid(rpwa_lik_ctx* c, int slot)
 {
 	int wgPerCu = 1;
-	int st = event_kernel_occupancy(c->lay, c->tile[rank], rank, &wgPerCu);
+	int st = event_kernel_occupancy(c->lay, c->tile[slot], std::min(slot, kMaxRank), &wgPerCu);
 	if (st != RPWA_OK) {
 		set_error("hipOccupancyMaxActiveBlocksPerMultiprocessor failed");
 		return st;
 	}
-	c->nWg[rank] = c->nCu * std::min(wgPerCu, 8);
-	// sized once for the largest grid any tile can get (8 workgroups per CU), so a
-	// tile change never moves the buffer under a kernel argument that still points at it
-	const size_t need = (size_t)c->nCu * 8 * partial_elems(kMaxRank, c->lay.wmax);
-	if (need > c->partialCap) {
-		hipFree(c->d_partials);
-		c->d_partials = nullptr;
-		RPWA_HIP_CHECK(hipMalloc(&c->d_partials, need * sizeof(double)));
-		c->partialCap = need;
-	}
+	c->nWg[slot] = c->nCu * std::min(wgPerCu, 8);
+	return RPWA_OK;
+}
+
+// Buffers of the evaluation sized for `rank` (at least kMaxRank): production
+// amplitudes and the staged kernel's T per row column for every rank block
+// (device and pinned host), the result row, the workgroup rows of the largest
+// grid any tile can get (8 workgroups per CU), and for ranks above kMaxRank the
+// per-event sigma.  Called only while no evaluation is in flight.
+int ensure_rank_buffers(rpwa_lik_ctx* c, uint32_t rank)
+{
+	const uint32_t r = std::max<uint32_t>(rank, (uint32_t)kMaxRank);
+	if (r > (uint32_t)kMaxRank && !c->d_sigma)
+		RPWA_HIP_CHECK(hipMalloc(&c->d_sigma, std::max<size_t>(c->lay.nEvents, 1) * sizeof(double)));
+	if (r <= c->rankCap)
+		return RPWA_OK;
+	const size_t blocks = (r + kMaxRank - 1) / kMaxRank;
+	const size_t nProd = (size_t)r * 2 * c->lay.wmax + blocks * c->lay.rowLen * kMaxRank;
+	const size_t nOut = partial_elems((int)r, c->lay.wmax);
+	const size_t need = (size_t)c->nCu * 8 * nOut;
+	hipFree(c->d_prod);
+	hipFree(c->d_partials);
+	hipHostFree(c->h_prod);
+	hipHostFree(c->h_out);
+	c->d_prod = nullptr;
+	c->d_partials = nullptr;
+	c->h_prod = nullptr;
+	c->h_out = nullptr;
+	c->d_out = nullptr;
+	c->rankCap = 0;
+	c->partialCap = 0;
+	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
+	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command)
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+	RPWA_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_out), c->h_out, 0));
+	RPWA_HIP_CHECK(hipMalloc(&c->d_partials, need * sizeof(double)));
+	c->partialCap = need;
+	c->rankCap = r;
 	return RPWA_OK;
 }
 
@@ -183,7 +217,7 @@ int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
 		          " waves in total)");
 		return RPWA_ERR_UNSUPPORTED;
 	}
-	for (int r = 0; r <= kMaxRank; ++r) {
+	for (int r = 0; r <= kHighRankSlot; ++r) {
 		c->tile[r] = ts;
 		c->tileFixed[r] = false;
 		c->evaluated[r] = false;
@@ -193,14 +227,8 @@ int setup_layout(rpwa_lik_ctx* c, uint64_t nEvents, const uint32_t nW[2])
 
 int alloc_eval_buffers(rpwa_lik_ctx* c)
 {
-	// production amplitudes [rank][2][wmax], followed by the staged kernel's T per row column
-	const size_t nProd = (size_t)kMaxRank * 2 * c->lay.wmax + (size_t)c->lay.rowLen * kMaxRank;
-	const size_t nOut = partial_elems(kMaxRank, c->lay.wmax);
-	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
-	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
-	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command)
-	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
-	RPWA_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_out), c->h_out, 0));
+	if (int st = ensure_rank_buffers(c, (uint32_t)kMaxRank))
+		return st;
 	for (int r = 1; r <= kMaxRank; ++r)
 		if (int st = configure_grid(c, r))
 			return st;
@@ -212,7 +240,6 @@ int load_from_device(rpwa_lik_ctx* c, const double* d_src, int srcLayout, uint64
                      const uint32_t nW[2], const double* waveNorm)
 {
 	free_buffers(c);
-	c->partialCap = 0;
 	int st = setup_layout(c, nEvents, nW);
 	if (st != RPWA_OK)
 		return st;
@@ -315,7 +342,9 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 
 // Start one evaluation on the context's stream; results are collected by
 // collect().  The whole result row (value, flat derivative, derivatives) is
-// always copied back: a few kB.
+// always copied back: a few kB.  Ranks up to kMaxRank: one fused event kernel;
+// higher ranks: the staged kernel in rank blocks (sigma of every block, then
+// the gradient entries of every block), then the same reduction.
 int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double flat, uint32_t rank,
            const char* fn)
 {
@@ -327,35 +356,56 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 		set_error(std::string(fn) + ": an evaluation is still pending (rpwa_lik_collect first)");
 		return RPWA_ERR_STATE;
 	}
-	if (!prodAmps || rank == 0 || rank > (uint32_t)kMaxRank) {
-		set_error(std::string(fn) + ": production amplitudes missing or rank outside 1.." +
-		          std::to_string(kMaxRank));
-		return rank > (uint32_t)kMaxRank ? RPWA_ERR_UNSUPPORTED : RPWA_ERR_INVALID;
+	if (!prodAmps || rank == 0) {
+		set_error(std::string(fn) + ": production amplitudes missing or rank 0");
+		return RPWA_ERR_INVALID;
 	}
 	const uint32_t wmax = c->lay.wmax;
-	if (nmbProdAmps != rank * 2 * wmax) {
+	if ((uint64_t)nmbProdAmps != (uint64_t)rank * 2 * wmax) {
 		set_error(std::string(fn) + ": nmbProdAmps must be rank*2*nmbWavesMax = " +
-		          std::to_string(rank * 2 * wmax));
+		          std::to_string((uint64_t)rank * 2 * wmax));
 		return RPWA_ERR_INVALID;
 	}
 	RPWA_HIP_CHECK(hipSetDevice(c->device));
-	if (!c->tileFixed[rank]) {
-		if (c->autotune)
+	if (int st = ensure_rank_buffers(c, rank))
+		return st;
+	const int slot = rank_slot(rank);
+	if (!c->tileFixed[slot]) {
+		if (slot == kHighRankSlot) {
+			// ranks above kMaxRank: the staged kernel with the largest tile that fits
+			TileShape ts;
+			if (!make_stage_tile(c->lay, kMaxRank, ts)) {
+				set_error(std::string(fn) + ": rank " + std::to_string(rank) + " needs the LDS-staged kernel, "
+				          "and the amplitude row does not fit it");
+				return RPWA_ERR_UNSUPPORTED;
+			}
+			c->tile[slot] = ts;
+			if (int st = configure_grid(c, slot))
+				return st;
+		} else if (c->autotune) {
 			if (int st = autotune_rank(c, (int)rank))
 				return st;
-		c->tileFixed[rank] = true;
+		}
+		c->tileFixed[slot] = true;
 	}
 	std::memcpy(c->h_prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2));
+	const bool blocks = rank > (uint32_t)kMaxRank;
+	const bool staged = c->tile[slot].events > 0;
+	const uint32_t nBlocks = blocks ? (rank + kMaxRank - 1) / kMaxRank : 1;
+	const int ranks = blocks ? kMaxRank : stage_rank_cap((int)rank);
 	size_t nUp = nmbProdAmps;
-	const bool staged = c->tile[rank].events > 0;
 	if (staged) {
-		const int ranks = stage_rank_cap((int)rank);
-		fill_tpad(c->lay, reinterpret_cast<const double2*>(c->h_prod), (int)rank, ranks,
-		          reinterpret_cast<double2*>(c->h_prod) + nmbProdAmps);
-		nUp += (size_t)c->lay.rowLen * ranks;
+		// T per row column, one [rowLen][ranks] array per rank block
+		for (uint32_t b = 0; b < nBlocks; ++b) {
+			const uint32_t r0 = b * kMaxRank;
+			const int nr = (int)std::min<uint32_t>(rank - r0, (uint32_t)ranks);
+			fill_tpad(c->lay, reinterpret_cast<const double2*>(c->h_prod) + (size_t)r0 * 2 * wmax, nr, ranks,
+			          reinterpret_cast<double2*>(c->h_prod) + nUp);
+			nUp += (size_t)c->lay.rowLen * ranks;
+		}
 	}
 	// the staged kernel takes T in its arguments when it fits: nothing to upload
-	if (!(staged && stage_t_inline(c->lay, (int)rank)))
+	if (!(staged && !blocks && stage_t_inline(c->lay, (int)rank)))
 		RPWA_HIP_CHECK(hipMemcpyAsync(c->d_prod, c->h_prod, nUp * sizeof(double2), hipMemcpyHostToDevice,
 		                              c->stream));
 	const uint32_t nElem = partial_elems((int)rank, wmax);
@@ -368,22 +418,35 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.nW0 = c->lay.nW[0];
 	a.nW1 = c->lay.nW[1];
 	a.wmax = wmax;
-	a.rank = (int)rank;
-	a.lanes0 = c->tile[rank].lanes0;
-	a.lanes1 = c->tile[rank].lanes1;
+	a.rank = blocks ? kMaxRank : (int)rank;
+	a.lanes0 = c->tile[slot].lanes0;
+	a.lanes1 = c->tile[slot].lanes1;
 	a.prod = reinterpret_cast<const double2*>(c->d_prod);
 	a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps;
-	a.tpadHost = reinterpret_cast<const double2*>(c->h_prod) + nmbProdAmps;
+	a.tpadHost = blocks ? nullptr : reinterpret_cast<const double2*>(c->h_prod) + nmbProdAmps;
 	a.flat2 = flat * flat;
 	a.partials = c->d_partials;
 	a.nElem = nElem;
+	a.sigma = c->d_sigma;
 	RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-	RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[rank], a, true, c->nWg[rank], c->stream));
+	if (!blocks) {
+		RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[slot], a, true, c->nWg[slot], c->stream));
+	} else {
+		const size_t tStride = (size_t)c->lay.rowLen * kMaxRank;
+		for (int pass = 0; pass < 2; ++pass)
+			for (uint32_t b = 0; b < nBlocks; ++b) {
+				a.stageMode = pass == 0 ? STAGE_SIGMA : STAGE_GRAD;
+				a.accumulate = b > 0 ? 1u : 0u;
+				a.rank0 = b * kMaxRank;
+				a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps + b * tStride;
+				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, a, c->nWg[slot], c->stream));
+			}
+	}
 	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
-	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, flat, c->d_out, c->stream));
+	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream));
 	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
 	c->pending = true;
-	c->evaluated[rank] = true;
+	c->evaluated[slot] = true;
 	c->pendingRank = rank;
 	c->pendingProd = nmbProdAmps;
 	return RPWA_OK;
@@ -705,7 +768,7 @@ extern "C" int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9])
 	info[2] = ctx->lay.nW[1];
 	info[3] = (uint64_t)ctx->precision;
 	int r = 1;
-	for (int k = kMaxRank; k >= 1; --k)
+	for (int k = kHighRankSlot; k >= 1; --k)
 		if (ctx->evaluated[k])
 			r = k;  // lowest rank evaluated so far (1 if none)
 	info[4] = (uint64_t)ctx->tile[r].lanes;
@@ -748,6 +811,15 @@ extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
 		if (int st = configure_grid(ctx, r))
 			return st;
 	}
+	// ranks above kMaxRank always run the staged kernel in rank blocks
+	TileShape high;
+	ctx->tileFixed[kHighRankSlot] = false;
+	if (make_stage_tile(ctx->lay, kMaxRank, high)) {
+		ctx->tile[kHighRankSlot] = high;
+		ctx->tileFixed[kHighRankSlot] = true;
+		if (int st = configure_grid(ctx, kHighRankSlot))
+			return st;
+	}
 	return RPWA_OK;
 	RPWA_GUARD_END
 }
@@ -775,8 +847,8 @@ extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile
 		}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
-	for (int r = 1; r <= kMaxRank; ++r) {
-		make_stage_tile(ctx->lay, r, ts, e);
+	for (int r = 1; r <= kHighRankSlot; ++r) {
+		make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts, e);
 		ctx->tile[r] = ts;
 		ctx->tileFixed[r] = true;
 		if (int st = configure_grid(ctx, r))

@@ -119,13 +119,13 @@ extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint3
 		set_error("rpwa_lik_hessian: decay amplitudes not loaded");
 		return RPWA_ERR_STATE;
 	}
-	if (!prodAmps || !hessian || !flatTerms || !hessianFlat || rank == 0 || rank > (uint32_t)kMaxRank) {
-		set_error("rpwa_lik_hessian: null output or rank outside 1.." + std::to_string(kMaxRank));
+	if (!prodAmps || !hessian || !flatTerms || !hessianFlat || rank == 0) {
+		set_error("rpwa_lik_hessian: null output or rank 0");
 		return RPWA_ERR_INVALID;
 	}
 	const AmpLayout& lay = ctx->lay;
 	const uint32_t wmax = lay.wmax, W = lay.nW[0] + lay.nW[1];
-	if (nmbProdAmps != rank * 2 * wmax) {
+	if ((uint64_t)nmbProdAmps != (uint64_t)rank * 2 * wmax) {
 		set_error("rpwa_lik_hessian: nmbProdAmps must be rank*2*nmbWavesMax");
 		return RPWA_ERR_INVALID;
 	}

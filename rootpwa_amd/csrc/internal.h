@@ -25,8 +25,16 @@ void clear_error();
 		}                                                                               \
 	} while (0)
 
-// Highest rank held in registers by the event kernels.
+// Highest rank held in registers by the event kernels; higher ranks run through
+// the LDS-staged kernel in blocks of kMaxRank ranks (one context slot for all of them).
 constexpr int kMaxRank = 4;
+constexpr int kHighRankSlot = kMaxRank + 1;
+inline int rank_slot(uint32_t rank) { return rank <= (uint32_t)kMaxRank ? (int)rank : kHighRankSlot; }
+
+// launch modes of the LDS-staged kernel
+constexpr int STAGE_FUSED = 0;  // all ranks in one launch: log sum, weight sum, gradient
+constexpr int STAGE_SIGMA = 1;  // one rank block: add its sum_r |S_r|^2 to the per-event sigma
+constexpr int STAGE_GRAD = 2;   // one rank block: gradient entries from the complete sigma
 
 // Device layout of a likelihood context's amplitudes: one row per event; the
 // row holds the refl -1 waves, then the refl +1 waves.  In mixed precision
@@ -82,6 +90,10 @@ struct EventArgs {
 	double flat2;
 	double* partials;      // [nWg][nElem]
 	uint32_t nElem;
+	int stageMode = STAGE_FUSED;  // the staged kernel's launch mode
+	uint32_t rank0 = 0;           // first rank of the block (STAGE_GRAD)
+	uint32_t accumulate = 0;      // STAGE_SIGMA: add to sigma
+	double* sigma = nullptr;      // per-event sigma of the rank blocks [nEvents]
 };
 
 // Number of doubles in a workgroup partial: lnL sum, lnL compensation,
@@ -120,15 +132,18 @@ struct rpwa_lik_ctx {
 	rpwa_amd::AmpLayout lay;
 	// tile and persistent grid per rank (index = rank); fixed before the first
 	// evaluation of that rank, so every result of a rank comes from one tile
-	rpwa_amd::TileShape tile[rpwa_amd::kMaxRank + 1];
-	int nWg[rpwa_amd::kMaxRank + 1] = {0, 0, 0, 0, 0};
-	bool tileFixed[rpwa_amd::kMaxRank + 1] = {false, false, false, false, false};
-	bool evaluated[rpwa_amd::kMaxRank + 1] = {false, false, false, false, false};
+	// (slot rank_slot(rank): ranks above kMaxRank share the last slot)
+	rpwa_amd::TileShape tile[rpwa_amd::kHighRankSlot + 1];
+	int nWg[rpwa_amd::kHighRankSlot + 1] = {0, 0, 0, 0, 0, 0};
+	bool tileFixed[rpwa_amd::kHighRankSlot + 1] = {false, false, false, false, false, false};
+	bool evaluated[rpwa_amd::kHighRankSlot + 1] = {false, false, false, false, false, false};
 	bool autotune = true;
 	bool loaded = false;
 	int nCu = 0;
 	void* d_amps = nullptr;
-	double* d_prod = nullptr;      // [kMaxRank][2][wmax] complex
+	uint32_t rankCap = 0;          // highest rank the buffers below are sized for
+	double* d_sigma = nullptr;     // [nEvents] per-event sigma of the rank blocks (ranks above kMaxRank)
+	double* d_prod = nullptr;      // [rankCap][2][wmax] complex, then the staged kernel's T per row column
 	double* d_partials = nullptr;  // [nWg][partial_elems]
 	double* d_out = nullptr;       // device view of h_out: the reduce kernel writes the results there
 	double* h_prod = nullptr;      // pinned staging

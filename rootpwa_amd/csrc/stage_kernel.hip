@@ -30,6 +30,15 @@
 //     event_reduce and everything after it is shared.
 // Every sum has a fixed order for a fixed grid: results are bitwise
 // reproducible and invariant under T -> -T of a (rank, sector) block.
+//
+// Ranks above the instantiation's R (at most 4 in registers) run in blocks of
+// R ranks over several launches (STAGE_SIGMA, then STAGE_GRAD per block):
+// sigma_e = sum over the blocks of sum_{r in block} |S_r(e)|^2 is accumulated
+// in a per-event device array, block by block in order, then every block's
+// launch reads the complete sigma_e, forms u = -2 S/sigma for its own ranks
+// and adds its gradient entries to the workgroup rows; the first block also
+// takes the log sum and the weight sum.  Each amplitude tile is read 2 x blocks
+// times; the sums keep a fixed order.
 #include "event_kernel.h"
 
 #include <cstddef>
@@ -62,6 +71,10 @@ __device__ __forceinline__ double2 scalar_t(cdouble* tp, uint32_t i)
 
 struct StageArgs {
 	const f32x4* amps;      // nEvents rows of rowUnits 16-byte units
+	int mode;               // STAGE_FUSED, STAGE_SIGMA or STAGE_GRAD
+	uint32_t rank0;         // first rank of this launch's block (STAGE_GRAD)
+	uint32_t accumulate;    // STAGE_SIGMA: add to sigma (blocks after the first)
+	double* sigma;          // STAGE_SIGMA / STAGE_GRAD: per-event sum over the ranks [nEvents]
 	uint64_t nEvents;
 	uint64_t lastUnit;      // nEvents * rowUnits - 1
 	uint32_t nTiles;
@@ -82,6 +95,8 @@ struct StageArgs {
 	uint32_t tInline;       // 1: T is tin below (read from the kernel arguments), not tpad
 	double2 tin[kInlineT];
 };
+
+static_assert(sizeof(StageArgs) <= 4096, "the staged kernel's arguments exceed the 4 KiB kernel-argument segment");
 
 // all LDS writes of this wave done, then the workgroup barrier (no wait for
 // the global loads in flight)
@@ -180,12 +195,22 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 			sigma += S[0][r].x * S[0][r].x + S[0][r].y * S[0][r].y;
 			sigma += S[1][r].x * S[1][r].x + S[1][r].y * S[1][r].y;
 		}
+		const uint64_t ev = (uint64_t)tile * E + lane;
+		const bool valid = ev < a.nEvents;
+		if (a.mode == STAGE_SIGMA) {
+			// this block's share of sigma_e, added in block order
+			if (valid)
+				a.sigma[ev] = a.accumulate ? a.sigma[ev] + sigma : sigma;
+		}
+		if (a.mode == STAGE_GRAD)
+			sigma = valid ? a.sigma[ev] : 0.;
 		sigma += a.flat2;
 		double w = 0.;
-		if ((uint64_t)tile * E + lane < a.nEvents) {
+		if (valid && a.mode != STAGE_SIGMA) {
 			// sum of ln sigma as the log of a running product m 2^e, m in [1, 2)
-			// (one log per lane at the end instead of one per event)
-			mant *= sigma;
+			// (one log per lane at the end instead of one per event); the rank
+			// blocks after the first leave it to the first
+			mant *= a.rank0 == 0 ? sigma : 1.;
 			const uint64_t ub = (uint64_t)__double_as_longlong(mant);
 			const int ex = (int)((ub >> 52) & 0x7FF);
 			if (ex != 0 && ex != 0x7FF) {
@@ -208,7 +233,7 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 	lds_barrier();
 
 	// ---- G phase: thread = (group of UG units of one sector, event subset) ----
-	if (gOn) {
+	if (gOn && a.mode != STAGE_SIGMA) {
 		// The slots of a group are consecutive units, read at fixed offsets from one
 		// pointer per event.  Slots past the sector (or the row) read the next units,
 		// the next row or the LDS areas after the stage, all inside the allocation;
@@ -328,7 +353,8 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		}
 	}
 #undef RPWA_STAGE_FETCH
-
+	if (a.mode == STAGE_SIGMA)
+		return;  // uniform: the whole workgroup leaves
 
 	// ---- workgroup partial row ----
 	lds_barrier();
@@ -367,7 +393,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 			ws = upper ? wo + ws : ws + wo;
 			s = t;
 		}
-		if (lane == 0) {
+		if (lane == 0 && a.rank0 == 0) {
 			out[0] = s;
 			out[1] = c;
 			out[2] = ws;
@@ -380,9 +406,12 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		const uint32_t j = idx % a.wmax;
 		const uint32_t rs = idx / a.wmax;
 		const int s = (int)(rs & 1);
-		const int r = (int)(rs >> 1);
+		const int rg = (int)(rs >> 1);  // rank of the element
+		if (rg < (int)a.rank0 || rg >= (int)a.rank0 + R)
+			continue;  // another rank block's element
+		const int r = rg - (int)a.rank0;
 		double2 v = make_double2(0., 0.);
-		if (r < R && j < (s ? a.nW1 : a.nW0)) {
+		if (j < (s ? a.nW1 : a.nW0)) {
 			const uint32_t c = (s ? a.secOff1 : 0u) + j;
 			const uint32_t unit = c / P;
 			const uint32_t k = c % P;
@@ -475,6 +504,10 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 		return hipErrorInvalidValue;
 	StageArgs a;
 	a.amps = reinterpret_cast<const f32x4*>(ea.amps);
+	a.mode = ea.stageMode;
+	a.rank0 = ea.rank0;
+	a.accumulate = ea.accumulate;
+	a.sigma = ea.sigma;
 	a.nEvents = lay.nEvents;
 	a.lastUnit = lay.nEvents ? lay.nEvents * p.rowUnits - 1 : 0;  // no tile, no load when there are no events
 	a.nTiles = (uint32_t)((lay.nEvents + events - 1) / events);

@@ -28,7 +28,9 @@ def build_pair(nW, rank, seed, prior=None, precision=0, nData=900, nMC=4000):
     return ds, Lo, Lg
 
 
-CASES = [((0, 16), 1, None), ((8, 80), 2, None), ((3, 5), 2, 0.5), ((7, 34), 2, None), ((4, 6), 3, 0.7)]
+# ranks above 4 run the staged kernel in blocks of 4 ranks (5 and 6: two blocks, 9: three)
+CASES = [((0, 16), 1, None), ((8, 80), 2, None), ((3, 5), 2, 0.5), ((7, 34), 2, None), ((4, 6), 3, 0.7),
+         ((3, 7), 5, None), ((6, 9), 6, 0.6), ((2, 12), 9, None)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: "W%d+%d_r%d_%s" % (c[0][0], c[0][1], c[1],
@@ -103,6 +105,23 @@ def test_mixed_precision_error(gpu):
     print("mixed |dlnL|", abs(f - f_ref), "grad rel", H.rel_err(g, g_ref))
     assert abs(f - f_ref) < 1e-3
     assert H.rel_err(g, g_ref) < 1e-5
+
+
+@pytest.mark.parametrize("rank", [5, 6])
+def test_high_rank_mixed_precision(gpu, rank):
+    """Ranks above 4 with complex64 storage: value and gradient against the
+    oracle, DoEval bitwise FdF, and bitwise invariance under the sign
+    convention."""
+    ds, Lo, Lg = build_pair((4, 9), rank, seed=60 + rank, precision=1, nData=2500)
+    rng = np.random.default_rng(rank)
+    par = rng.normal(size=Lo.nPars)
+    f_ref, g_ref = Lo.fdf(par)
+    f, g = Lg.FdF(par)
+    print("rank", rank, "mixed |dlnL|", abs(f - f_ref), "grad rel", H.rel_err(g, g_ref))
+    assert abs(f - f_ref) < 1e-6 * max(1.0, abs(f_ref))
+    assert H.rel_err(g, g_ref) < 1e-5
+    assert Lg.DoEval(par) == f and np.array_equal(Lg.Gradient(par), g)
+    assert Lg.DoEval(Lg.CorrectParamSigns(par)) == f
 
 
 def test_integral_matrix_class(gpu, tmp_path):
