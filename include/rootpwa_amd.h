@@ -148,7 +148,9 @@ int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
  * through the scalar unit; rpwa_lik_set_tile(ctx, 1) takes the largest tile
  * that fits).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
  * environment) keeps the modelled default instead; rpwa_lik_set_tile fixes the
- * variant of every rank (RPWA_ERR_UNSUPPORTED if the row does not fit it). */
+ * variant of every rank up to 4 (RPWA_ERR_UNSUPPORTED if the row does not fit it);
+ * ranks above 4 always run the LDS-staged kernel (the largest tile that fits, or
+ * the tile fixed by rpwa_lik_set_stage_tile). */
 int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable);
 int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
 /* Fixes the LDS-staged kernel with tiles of eventsPerTile (16, 32 or 64)
