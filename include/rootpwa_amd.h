@@ -187,9 +187,12 @@ int rpwa_dev_info(int device, char* name, int nameLen, int* nCu, uint64_t* total
 int rpwa_wave_nonzero_device(int device, const double* d_amps, uint64_t nmbEvents, uint32_t nmbWaves,
                              uint32_t wave, int* nonzero);
 
-/* HBM bandwidth probe: median GB/s of a streaming read and of a copy (read +
- * write bytes) over `bytes` of device memory; the counterpart of the
- * reference's bandwidth microbenchmarks (cuda/test/benchmarkMemoryBandwidth.cu). */
+/* HBM bandwidth probe: GB/s of a streaming read and of a copy (read + write
+ * bytes) over `bytes` of device memory, each the best median over several
+ * variants (4 to 16 16-byte loads in flight per thread, temporal and
+ * non-temporal loads, 4 to 16 workgroups per compute unit): the ceiling a
+ * streaming kernel reaches on the device.  The counterpart of the reference's
+ * bandwidth microbenchmarks (cuda/test/benchmarkMemoryBandwidth.cu:85-260). */
 int rpwa_bandwidth_probe(int device, uint64_t bytes, int repetitions, double* readGBs, double* copyGBs);
 
 /* ---------------------------------------------------------------------------

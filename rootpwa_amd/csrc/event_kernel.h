@@ -48,11 +48,21 @@ __device__ __forceinline__ void cmac_conj(double2& acc, const double2 u, const d
 template <bool MIXED>
 struct SlotData;
 
+// Amplitude loads are non-temporal: the array is streamed once per evaluation
+// and is far larger than the caches (streaming reads reach ~7 TB/s this way on
+// an MI355X, ~6.2 TB/s through the caches).
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+
 template <>
 struct SlotData<false> {
 	double2 v;
 	__device__ __forceinline__ void zero() { v = make_double2(0., 0.); }
-	__device__ __forceinline__ void load(const void* p) { v = *reinterpret_cast<const double2*>(p); }
+	__device__ __forceinline__ void load(const void* p)
+	{
+		const nt_d2 x = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+		v = make_double2(x.x, x.y);
+	}
 	__device__ __forceinline__ double2 get(int) const { return v; }
 };
 
@@ -60,7 +70,11 @@ template <>
 struct SlotData<true> {
 	float4 v;
 	__device__ __forceinline__ void zero() { v = make_float4(0.f, 0.f, 0.f, 0.f); }
-	__device__ __forceinline__ void load(const void* p) { v = *reinterpret_cast<const float4*>(p); }
+	__device__ __forceinline__ void load(const void* p)
+	{
+		const nt_f4 x = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+		v = make_float4(x.x, x.y, x.z, x.w);
+	}
 	__device__ __forceinline__ double2 get(int p) const
 	{
 		return p == 0 ? make_double2((double)v.x, (double)v.y) : make_double2((double)v.z, (double)v.w);

@@ -65,6 +65,8 @@ struct TileShape {
 // Geometry of the LDS-staged kernel (stage_kernel.hip) for one layout, tile size and rank.
 struct StagePlan {
 	int events = 0;
+	int threads = 256;       // workgroup size
+	uint32_t sliceUnits = 0; // units per wave in the S phase
 	int ranks = 1;           // ranks held by the kernel instantiation (1, 2 or 4)
 	uint32_t rowUnits = 0;   // 16-byte units per amplitude row
 	uint32_t rowUnitsP = 0;  // LDS row stride (odd)

@@ -115,19 +115,16 @@ __device__ __forceinline__ double2 unit_cplx(const f32x4 v, int k)
 }
 
 // The three phases of one staged tile: S (all waves), sigma (wave 0), G (all
-// threads).  The stage holds the tile; part is the LDS exchange area.
+// threads).  The stage holds the tile; part holds the S partials of waves
+// 1..NW-1 ([NW-1][E][PS]); u is the exchange area of u = -2 S / sigma ([E][PS]).
+
+// ---- S phase: wave wid, slice of units [u0, u1), lane = event ----
 template <bool MIXED, int R, int E>
-__device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* stage, double2* part, int lane,
-                                            int wid, uint32_t tile, bool gOn, uint32_t gU0, uint32_t gSub,
-                                            int gSec,
-                                            double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R],
-                                            double& mant, int& expo, double& wsum)
+__device__ __forceinline__ void s_phase(const StageArgs& a, const f32x4* stage, double2* part, int lane, int wid,
+                                        double2 (&S)[2][R])
 {
 	constexpr int P = MIXED ? 2 : 1;
-	constexpr int UG = group_units<MIXED, R>();
 	constexpr int PS = part_stride<R>();
-	// ---- S phase: wave wid, slice of units [u0, u1), lane = event ----
-	double2 S[2][R];
 #pragma unroll
 	for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -173,190 +170,118 @@ __device__ __forceinline__ void tile_phases(const StageArgs& a, const f32x4* sta
 					p[s * R + r] = S[s][r];
 		}
 	}
-	lds_barrier();
+}
 
-	// ---- sigma phase: wave 0, lane = event ----
-	if (wid == 0 && (E == 64 || lane < E)) {
+// ---- sigma phase: wave 0, lane = event; S holds wave 0's own partial ----
+template <bool MIXED, int R, int E, int NW>
+__device__ __forceinline__ void sigma_phase(const StageArgs& a, const double2* part, double2* uOut, int lane,
+                                            uint32_t tile, double2 (&S)[2][R], double& mant, int& expo,
+                                            double& wsum)
+{
+	constexpr int PS = part_stride<R>();
+	if (!(E == 64 || lane < E))
+		return;
 #pragma unroll
-		for (int w = 1; w < kWaves; ++w) {
-			const double2* p = part + ((size_t)(w - 1) * E + lane) * PS;
-#pragma unroll
-			for (int s = 0; s < 2; ++s)
-#pragma unroll
-				for (int r = 0; r < R; ++r) {
-					const double2 q = p[s * R + r];
-					S[s][r].x += q.x;
-					S[s][r].y += q.y;
-				}
-		}
-		double sigma = 0.;
-#pragma unroll
-		for (int r = 0; r < R; ++r) {
-			sigma += S[0][r].x * S[0][r].x + S[0][r].y * S[0][r].y;
-			sigma += S[1][r].x * S[1][r].x + S[1][r].y * S[1][r].y;
-		}
-		const uint64_t ev = (uint64_t)tile * E + lane;
-		const bool valid = ev < a.nEvents;
-		if (a.mode == STAGE_SIGMA) {
-			// this block's share of sigma_e, added in block order
-			if (valid)
-				a.sigma[ev] = a.accumulate ? a.sigma[ev] + sigma : sigma;
-		}
-		if (a.mode == STAGE_GRAD)
-			sigma = valid ? a.sigma[ev] : 0.;
-		sigma += a.flat2;
-		double w = 0.;
-		if (valid && a.mode != STAGE_SIGMA) {
-			// sum of ln sigma as the log of a running product m 2^e, m in [1, 2)
-			// (one log per lane at the end instead of one per event); the rank
-			// blocks after the first leave it to the first
-			mant *= a.rank0 == 0 ? sigma : 1.;
-			const uint64_t ub = (uint64_t)__double_as_longlong(mant);
-			const int ex = (int)((ub >> 52) & 0x7FF);
-			if (ex != 0 && ex != 0x7FF) {
-				expo += ex - 1023;
-				mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
-			}
-			// -2/sigma from the hardware reciprocal and one Newton step
-			double r = __builtin_amdgcn_rcp(sigma);
-			r = fma(fma(-sigma, r, 1.), r, r);
-			w = -2. * r;
-			wsum += w;
-		}
-		double2* p = part + (size_t)lane * PS;  // u replaces wave 1's partial
+	for (int w = 1; w < NW; ++w) {
+		const double2* p = part + ((size_t)(w - 1) * E + lane) * PS;
 #pragma unroll
 		for (int s = 0; s < 2; ++s)
 #pragma unroll
-			for (int r = 0; r < R; ++r)
-				p[s * R + r] = make_double2(w * S[s][r].x, w * S[s][r].y);
+			for (int r = 0; r < R; ++r) {
+				const double2 q = p[s * R + r];
+				S[s][r].x += q.x;
+				S[s][r].y += q.y;
+			}
 	}
-	lds_barrier();
+	double sigma = 0.;
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		sigma += S[0][r].x * S[0][r].x + S[0][r].y * S[0][r].y;
+		sigma += S[1][r].x * S[1][r].x + S[1][r].y * S[1][r].y;
+	}
+	const uint64_t ev = (uint64_t)tile * E + lane;
+	const bool valid = ev < a.nEvents;
+	if (a.mode == STAGE_SIGMA) {
+		// this block's share of sigma_e, added in block order
+		if (valid)
+			a.sigma[ev] = a.accumulate ? a.sigma[ev] + sigma : sigma;
+	}
+	if (a.mode == STAGE_GRAD)
+		sigma = valid ? a.sigma[ev] : 0.;
+	sigma += a.flat2;
+	double w = 0.;
+	if (valid && a.mode != STAGE_SIGMA) {
+		// sum of ln sigma as the log of a running product m 2^e, m in [1, 2)
+		// (one log per lane at the end instead of one per event); the rank
+		// blocks after the first leave it to the first
+		mant *= a.rank0 == 0 ? sigma : 1.;
+		const uint64_t ub = (uint64_t)__double_as_longlong(mant);
+		const int ex = (int)((ub >> 52) & 0x7FF);
+		if (ex != 0 && ex != 0x7FF) {
+			expo += ex - 1023;
+			mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+		}
+		// -2/sigma from the hardware reciprocal and one Newton step
+		double r = __builtin_amdgcn_rcp(sigma);
+		r = fma(fma(-sigma, r, 1.), r, r);
+		w = -2. * r;
+		wsum += w;
+	}
+	double2* p = uOut + (size_t)lane * PS;
+#pragma unroll
+	for (int s = 0; s < 2; ++s)
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			p[s * R + r] = make_double2(w * S[s][r].x, w * S[s][r].y);
+}
 
-	// ---- G phase: thread = (group of UG units of one sector, event subset) ----
-	if (gOn && a.mode != STAGE_SIGMA) {
-		// The slots of a group are consecutive units, read at fixed offsets from one
-		// pointer per event.  Slots past the sector (or the row) read the next units,
-		// the next row or the LDS areas after the stage, all inside the allocation;
-		// their sums are discarded.
-		const f32x4* ap = stage + gSub * a.rowUnitsP + gU0;
-		const double2* up = part + gSub * PS + gSec * R;
-		const uint32_t aStep = a.gSubs * a.rowUnitsP, uStep = a.gSubs * PS;
-		for (uint32_t e = gSub; e < (uint32_t)E; e += a.gSubs, ap += aStep, up += uStep) {
-			double2 u[R];
+// ---- G phase: thread = (group of UG units of one sector, event subset) ----
+template <bool MIXED, int R, int E>
+__device__ __forceinline__ void g_phase(const StageArgs& a, const f32x4* stage, const double2* uIn, bool gOn,
+                                        uint32_t gU0, uint32_t gSub, int gSec,
+                                        double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R])
+{
+	constexpr int P = MIXED ? 2 : 1;
+	constexpr int UG = group_units<MIXED, R>();
+	constexpr int PS = part_stride<R>();
+	if (!gOn || a.mode == STAGE_SIGMA)
+		return;
+	// The slots of a group are consecutive units, read at fixed offsets from one
+	// pointer per event.  Slots past the sector (or the row) read the next units,
+	// the next row or the LDS areas after the stage, all inside the allocation;
+	// their sums are discarded.
+	const f32x4* ap = stage + gSub * a.rowUnitsP + gU0;
+	const double2* up = uIn + gSub * PS + gSec * R;
+	const uint32_t aStep = a.gSubs * a.rowUnitsP, uStep = a.gSubs * PS;
+	for (uint32_t e = gSub; e < (uint32_t)E; e += a.gSubs, ap += aStep, up += uStep) {
+		double2 u[R];
 #pragma unroll
-			for (int r = 0; r < R; ++r)
-				u[r] = up[r];
+		for (int r = 0; r < R; ++r)
+			u[r] = up[r];
 #pragma unroll
-			for (int j = 0; j < UG; ++j) {
-				const f32x4 v = ap[j];
+		for (int j = 0; j < UG; ++j) {
+			const f32x4 v = ap[j];
 #pragma unroll
-				for (int k = 0; k < P; ++k) {
-					const double2 x = unit_cplx<MIXED>(v, k);
+			for (int k = 0; k < P; ++k) {
+				const double2 x = unit_cplx<MIXED>(v, k);
 #pragma unroll
-					for (int r = 0; r < R; ++r)
-						ek::cmac_conj(G[j][k][r], u[r], x);
-				}
+				for (int r = 0; r < R; ++r)
+					ek::cmac_conj(G[j][k][r], u[r], x);
 			}
 		}
 	}
 }
 
-template <bool MIXED, int R, int E>
-__global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
+// ---- end: the workgroup's partial row (lnL sum and its compensation, weight
+// sum, gradient entries), in the format of the register-tile kernels ----
+template <bool MIXED, int R, int E, int NT>
+__device__ __forceinline__ void write_partial_row(const StageArgs& a, f32x4* smem, int tid, int lane, int wid, bool gOn,
+                                                  uint32_t gU0, uint32_t gEnd, uint32_t gSub,
+                                                  double2 (&G)[group_units<MIXED, R>()][MIXED ? 2 : 1][R],
+                                                  double mant, int expo, double wsum)
 {
-	constexpr int P = MIXED ? 2 : 1;  // complex values per 16-byte unit
-	constexpr int PS = part_stride<R>();
-	extern __shared__ f32x4 smem[];
-	f32x4* stage = smem;  // [E][rowUnitsP] units, then the dump unit
-	double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(smem) + a.stageBytes);  // [3][E][PS]
-
-	const int tid = threadIdx.x;
-	const int lane = tid & 63;
-	const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
-
-	// LDS positions of this thread's loads (the same for every tile); a load
-	// past the tile goes to the dump unit that follows the stage
-	const uint32_t dumpUnit = (uint32_t)E * a.rowUnitsP;
-	uint32_t ldsPos[kMaxLoads];
-	uint32_t ldOff[kMaxLoads];
-#pragma unroll
-	for (int k = 0; k < kMaxLoads; ++k) {
-		const uint32_t g = (uint32_t)tid + (uint32_t)kThreads * k;
-		const uint32_t row = g / a.rowUnits;
-		const bool in = g < a.tileUnits;
-		ldsPos[k] = in ? row * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
-		ldOff[k] = in ? g * 16u : 0u;  // byte offset in the tile (a load past the tile reads its first unit)
-	}
-
-	// G-phase role: unit group gi of sector gSec, units [gU0, min(gU0 + UG, gEnd))
+	constexpr int P = MIXED ? 2 : 1;
 	constexpr int UG = group_units<MIXED, R>();
-	const uint32_t gi = (uint32_t)tid % a.gGroups;
-	const uint32_t gSub = (uint32_t)tid / a.gGroups;
-	const bool gOn = gSub < a.gSubs;
-	const int gSec = gi >= a.gGroups0 ? 1 : 0;
-	const uint32_t gU0 = gSec ? a.secUnit + (gi - a.gGroups0) * UG : gi * UG;
-	const uint32_t gEnd = gSec ? a.rowUnits : a.secUnit;
-
-	double2 G[UG][P][R];
-#pragma unroll
-	for (int j = 0; j < UG; ++j)
-#pragma unroll
-		for (int k = 0; k < P; ++k)
-#pragma unroll
-			for (int r = 0; r < R; ++r)
-				G[j][k][r] = make_double2(0., 0.);
-	double mant = 1., wsum = 0.;
-	int expo = 0;
-
-	// Every thread issues kMaxLoads loads per tile, so that the staging registers
-	// are a fixed set: a load past the tile reads the tile's first unit (the
-	// tiles before the last are complete, so it stays inside the array), and in
-	// the last tile indices past the array are clamped to its last unit (rows past the last
-	// event hold copies of real amplitudes and get weight zero).  The loads of
-	// the next tile are in flight while a tile is computed (the fetch after the
-	// block's last tile re-reads the array's last tile, so that every stage
-	// write waits for the same loads).
-	f32x4 buf[kMaxLoads];
-	// Tiles before the array's last are complete: their loads are a scalar tile
-	// base plus the thread's fixed byte offsets; the last tile clamps its indices.
-#define RPWA_STAGE_FETCH(T_)                                                           \
-	do {                                                                               \
-		const uint32_t t_ = (T_);                                                      \
-		if (t_ < lastTile) {                                                           \
-			const char* tb_ = reinterpret_cast<const char*>(a.amps + (uint64_t)t_ * a.tileUnits); \
-			_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k)                      \
-				buf[k] = *reinterpret_cast<const f32x4*>(tb_ + ldOff[k]);              \
-		} else {                                                                       \
-			const uint64_t base_ = (uint64_t)t_ * a.tileUnits;                        \
-			_Pragma("unroll") for (int k = 0; k < kMaxLoads; ++k) {                    \
-				const uint64_t g_ = base_ + ldOff[k] / 16u;                            \
-				buf[k] = a.amps[g_ < a.lastUnit ? g_ : a.lastUnit];                    \
-			}                                                                          \
-		}                                                                              \
-	} while (0)
-
-	const uint32_t step = gridDim.x;
-	const uint32_t lastTile = a.nTiles ? a.nTiles - 1 : 0;
-	uint32_t tile = blockIdx.x;
-	if (tile < a.nTiles) {
-		RPWA_STAGE_FETCH(tile);
-		const uint32_t nMine = (a.nTiles - 1 - tile) / step + 1;  // tiles of this workgroup
-		for (uint32_t i = 0; i < nMine; ++i, tile += step) {
-			lds_barrier();  // the previous tile's G phase is done with the stage
-#pragma unroll
-			for (int k = 0; k < kMaxLoads; ++k)
-				stage[ldsPos[k]] = buf[k];
-			RPWA_STAGE_FETCH(min(tile + step, lastTile));
-			lds_barrier();
-			tile_phases<MIXED, R, E>(a, stage, part, lane, wid, tile, gOn, gU0, gSub, gSec, G, mant, expo,
-			                         wsum);
-		}
-	}
-#undef RPWA_STAGE_FETCH
-	if (a.mode == STAGE_SIGMA)
-		return;  // uniform: the whole workgroup leaves
-
-	// ---- workgroup partial row ----
 	lds_barrier();
 	double2* gbuf = reinterpret_cast<double2*>(smem);  // [gSubs][rowUnits][P][R]
 	if (gOn) {
@@ -402,7 +327,7 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 	lds_barrier();
 	// element 3 + 2*((r*2 + s)*wmax + j) (+1 for the imaginary part)
 	const uint32_t nCplx = (a.nElem - 3) / 2;
-	for (uint32_t idx = tid; idx < nCplx; idx += kThreads) {
+	for (uint32_t idx = tid; idx < nCplx; idx += NT) {
 		const uint32_t j = idx % a.wmax;
 		const uint32_t rs = idx / a.wmax;
 		const int s = (int)(rs & 1);
@@ -424,6 +349,124 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		out[3 + 2 * idx] = v.x;
 		out[3 + 2 * idx + 1] = v.y;
 	}
+}
+
+// Loads of one tile into the staging registers buf[KL]: tiles before the
+// array's last are complete, so their loads are the tile base plus each
+// thread's fixed byte offsets (a load past the tile reads the tile's first
+// unit); in the last tile indices past the array are clamped to its last unit
+// (rows past the last event hold copies of real amplitudes and get weight zero).
+// The loads are non-temporal: the array is streamed once per evaluation and is
+// far larger than the caches (streaming reads reach ~7 TB/s this way on an
+// MI355X, ~6.2 TB/s through the caches).
+template <int KL>
+__device__ __forceinline__ void fetch_tile(const StageArgs& a, uint32_t t, uint32_t lastTile, const uint32_t (&ldOff)[KL],
+                                           f32x4 (&buf)[KL])
+{
+	if (t < lastTile) {
+		const char* tb = reinterpret_cast<const char*>(a.amps + (uint64_t)t * a.tileUnits);
+#pragma unroll
+		for (int k = 0; k < KL; ++k)
+			buf[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tb + ldOff[k]));
+	} else {
+		const uint64_t base = (uint64_t)t * a.tileUnits;
+#pragma unroll
+		for (int k = 0; k < KL; ++k) {
+			const uint64_t g = base + ldOff[k] / 16u;
+			buf[k] = __builtin_nontemporal_load(a.amps + (g < a.lastUnit ? g : a.lastUnit));
+		}
+	}
+}
+
+// LDS positions and tile byte offsets of a thread's KL loads (the same for
+// every tile); a load past the tile goes to the dump unit after the stage and
+// reads the tile's first unit
+template <int NT, int KL, int E>
+__device__ __forceinline__ void load_plan(const StageArgs& a, int tid, uint32_t (&ldsPos)[KL], uint32_t (&ldOff)[KL])
+{
+	const uint32_t dumpUnit = (uint32_t)E * a.rowUnitsP;
+#pragma unroll
+	for (int k = 0; k < KL; ++k) {
+		const uint32_t g = (uint32_t)tid + (uint32_t)NT * k;
+		const uint32_t row = g / a.rowUnits;
+		const bool in = g < a.tileUnits;
+		ldsPos[k] = in ? row * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
+		ldOff[k] = in ? g * 16u : 0u;
+	}
+}
+
+template <int KL>
+__device__ __forceinline__ void store_tile(f32x4* stage, const uint32_t (&ldsPos)[KL], const f32x4 (&buf)[KL])
+{
+#pragma unroll
+	for (int k = 0; k < KL; ++k)
+		stage[ldsPos[k]] = buf[k];
+}
+
+// The staged kernel: 4 waves, one tile stage; per tile: stage the prefetched
+// tile, S, sigma, G, each phase behind a workgroup barrier.
+template <bool MIXED, int R, int E>
+__global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
+{
+	constexpr int P = MIXED ? 2 : 1;  // complex values per 16-byte unit
+	constexpr int UG = group_units<MIXED, R>();
+	extern __shared__ f32x4 smem[];
+	f32x4* stage = smem;  // [E][rowUnitsP] units, then the dump unit
+	double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(smem) + a.stageBytes);  // [3][E][PS]
+
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+	uint32_t ldsPos[kMaxLoads], ldOff[kMaxLoads];
+	load_plan<kThreads, kMaxLoads, E>(a, tid, ldsPos, ldOff);
+
+	// G-phase role: unit group gi of sector gSec, units [gU0, min(gU0 + UG, gEnd))
+	const uint32_t gi = (uint32_t)tid % a.gGroups;
+	const uint32_t gSub = (uint32_t)tid / a.gGroups;
+	const bool gOn = gSub < a.gSubs;
+	const int gSec = gi >= a.gGroups0 ? 1 : 0;
+	const uint32_t gU0 = gSec ? a.secUnit + (gi - a.gGroups0) * UG : gi * UG;
+	const uint32_t gEnd = gSec ? a.rowUnits : a.secUnit;
+
+	double2 G[UG][P][R];
+#pragma unroll
+	for (int j = 0; j < UG; ++j)
+#pragma unroll
+		for (int k = 0; k < P; ++k)
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				G[j][k][r] = make_double2(0., 0.);
+	double mant = 1., wsum = 0.;
+	int expo = 0;
+
+	// Every thread issues kMaxLoads loads per tile, so that the staging registers
+	// are a fixed set.  The loads of the next tile are in flight while a tile is
+	// computed (the fetch after the block's last tile re-reads the array's last
+	// tile, so that every stage write waits for the same loads).
+	f32x4 buf[kMaxLoads];
+	const uint32_t step = gridDim.x;
+	const uint32_t lastTile = a.nTiles ? a.nTiles - 1 : 0;
+	uint32_t tile = blockIdx.x;
+	if (tile < a.nTiles) {
+		fetch_tile(a, tile, lastTile, ldOff, buf);
+		const uint32_t nMine = (a.nTiles - 1 - tile) / step + 1;  // tiles of this workgroup
+		for (uint32_t i = 0; i < nMine; ++i, tile += step) {
+			lds_barrier();  // the previous tile's G phase is done with the stage
+			store_tile(stage, ldsPos, buf);
+			fetch_tile(a, min(tile + step, lastTile), lastTile, ldOff, buf);
+			lds_barrier();
+			double2 S[2][R];
+			s_phase<MIXED, R, E>(a, stage, part, lane, wid, S);
+			lds_barrier();
+			if (wid == 0)
+				sigma_phase<MIXED, R, E, kWaves>(a, part, part, lane, tile, S, mant, expo, wsum);  // u replaces wave 1's partial
+			lds_barrier();
+			g_phase<MIXED, R, E>(a, stage, part, gOn, gU0, gSub, gSec, G);
+		}
+	}
+	if (a.mode == STAGE_SIGMA)
+		return;  // uniform: the whole workgroup leaves
+	write_partial_row<MIXED, R, E, kThreads>(a, smem, tid, lane, wid, gOn, gU0, gEnd, gSub, G, mant, expo, wsum);
 }
 
 using StageKernel = void (*)(StageArgs);
@@ -466,12 +509,16 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 {
 	if (events != 16 && events != 32 && events != 64)
 		return false;
+	const int threads = kThreads;
+	const int waves = kWaves;
+	p.threads = threads;
 	const uint32_t bpc = lay.precision == RPWA_PREC_MIXED ? 8u : 16u;
 	const uint32_t P = 16u / bpc;
 	p.events = events;
 	p.rowUnits = lay.rowLen * bpc / 16u;
 	if (p.rowUnits == 0 || p.rowUnits > (uint32_t)kThreads)
 		return false;
+	p.sliceUnits = (p.rowUnits + waves - 1) / waves;
 	const uint32_t tileUnits = (uint32_t)events * p.rowUnits;
 	if (tileUnits > (uint32_t)(kThreads * kMaxLoads))
 		return false;
@@ -485,12 +532,13 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 	const uint32_t secUnit = lay.secOff[1] * bpc / 16u;
 	p.gGroups0 = (secUnit + ug - 1) / ug;
 	p.gGroups = p.gGroups0 + (p.rowUnits - secUnit + ug - 1) / ug;
-	if (p.gGroups > (uint32_t)kThreads)
+	if (p.gGroups > (uint32_t)threads)
 		return false;
-	p.gSubs = (uint32_t)kThreads / p.gGroups;
+	p.gSubs = (uint32_t)threads / p.gGroups;
 	p.stageBytes = ((uint32_t)events * p.rowUnitsP + 1u) * 16u;
 	const int ps = p.ranks == 1 ? part_stride<1>() : p.ranks == 2 ? part_stride<2>() : part_stride<4>();
-	const size_t partBytes = (size_t)(kWaves - 1) * events * ps * sizeof(double2);
+	// S partials of waves 1..3 (u replaces wave 1's)
+	const size_t partBytes = (size_t)(waves - 1) * events * ps * sizeof(double2);
 	const size_t endBytes = (size_t)p.gSubs * p.rowUnits * P * p.ranks * sizeof(double2) +
 	                        (size_t)events * 3 * sizeof(double);
 	p.ldsBytes = std::max((size_t)p.stageBytes + partBytes, endBytes);
@@ -514,7 +562,7 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 	a.rowUnits = p.rowUnits;
 	a.rowUnitsP = p.rowUnitsP;
 	a.tileUnits = (uint32_t)events * p.rowUnits;
-	a.sliceUnits = (p.rowUnits + kWaves - 1) / kWaves;
+	a.sliceUnits = p.sliceUnits;
 	const uint32_t bpc = lay.precision == RPWA_PREC_MIXED ? 8u : 16u;
 	a.secUnit = lay.secOff[1] * bpc / 16u;
 	a.gGroups0 = p.gGroups0;
@@ -543,7 +591,7 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 		if (e != hipSuccess)
 			return e;
 	}
-	hipLaunchKernelGGL(k, dim3(nWg), dim3(kThreads), p.ldsBytes, s, a);
+	hipLaunchKernelGGL(k, dim3(nWg), dim3(p.threads), p.ldsBytes, s, a);
 	return hipGetLastError();
 }
 
@@ -558,7 +606,7 @@ int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPe
 		                        (int)p.ldsBytes) != hipSuccess)
 			return RPWA_ERR_HIP;
 	int n = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, p.ldsBytes) != hipSuccess)
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, p.threads, p.ldsBytes) != hipSuccess)
 		return RPWA_ERR_HIP;
 	*wgPerCu = n > 0 ? n : 1;
 	return RPWA_OK;
