@@ -243,6 +243,8 @@ def main():
     # ---- timed FdF evaluations ------------------------------------------------
     def timed(L, label):
         ctx = L.gpuContext()
+        # HIP events around every evaluation in the timed region: the kernel durations of the roofline
+        ctx.set_timing(True)
         for _ in range(args.warmup):
             L.FdF(par)
         capi.synchronize()
@@ -268,8 +270,16 @@ def main():
             L.Gradient(par)
         el_pair = time.perf_counter() - t
         log("%s: %.1f DoEval+Gradient pairs/s" % (label, n_pair / el_pair))
+        # the library's default mode (no events; completion read from the reduction's flag)
+        ctx.set_timing(False)
+        capi.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            L.FdF(par)
+        rate_default = args.steps / (time.perf_counter() - t)
+        log("%s: %.1f evals/s without kernel events" % (label, rate_default))
         return dict(rate=rate, ms=1e3 * el / args.steps, kernel_s=k_avg, reduce_s=red / max(n, 1), f=f, g=g,
-                    info=info, pair_rate=n_pair / el_pair)
+                    info=info, pair_rate=n_pair / el_pair, rate_without_events=rate_default)
 
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
@@ -410,6 +420,7 @@ def fit_leg(L_64, L_mix, log):
     from rootpwa_amd import likelihood as PL
     out = {}
     for key, L, fn in (("fp64", L_64, "pwaFit"), ("fp64_nlopt", L_64, "pwaNloptFit"), ("mixed", L_mix, "pwaNloptFit")):
+        L.gpuContext().set_timing(True)
         L.gpuContext().reset_kernel_time()
         t = time.perf_counter()
         # each driver draws its random start values from the seed, as the reference's drivers do; the

@@ -122,9 +122,15 @@ int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProd
                      double* hessianFlat);
 
 /* Accumulated time spent in this context's kernels [s]
- * (likelihoodInterface::kernelTime / resetKernelTime, likelihoodInterface.cuh:124-125). */
+ * (likelihoodInterface::kernelTime / resetKernelTime, likelihoodInterface.cuh:124-125),
+ * measured with HIP events around every evaluation while timing is on
+ * (rpwa_lik_set_timing, or RPWA_KERNEL_TIMING=1 in the environment; off by
+ * default: the events cost host time on every evaluation, and with timing off
+ * an evaluation's completion is read from a flag the reduction writes). */
 int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds);
 int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx);
+/* Kernel timing on (enable != 0) or off for the following evaluations. */
+int rpwa_lik_set_timing(rpwa_lik_ctx* ctx, int enable);
 
 /* Split of the accumulated kernel time: event kernel and fixed-order
  * reduction [s], and the number of evaluations they cover. */

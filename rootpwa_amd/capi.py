@@ -53,6 +53,7 @@ PROTOTYPES = {
     "rpwa_lik_collect": (_i, [_vp, _dp, _dp, _dp]),
     "rpwa_lik_kernel_time": (_i, [_vp, _dp]),
     "rpwa_lik_reset_kernel_time": (_i, [_vp]),
+    "rpwa_lik_set_timing": (_i, [_vp, _i]),
     "rpwa_lik_kernel_times": (_i, [_vp, _dp, _dp, _u64p]),
     "rpwa_lik_info": (_i, [_vp, _u64p]),
     "rpwa_lik_set_tile": (_i, [_vp, _u32]),
@@ -230,6 +231,9 @@ class LikelihoodContext:
         a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
         call("rpwa_lik_kernel_times", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n))
         return a.value, b.value, n.value
+
+    def set_timing(self, enable=True):
+        call("rpwa_lik_set_timing", self.h, int(bool(enable)))
 
     def reset_kernel_time(self):
         call("rpwa_lik_reset_kernel_time", self.h)

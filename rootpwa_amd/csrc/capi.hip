@@ -12,6 +12,8 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -130,8 +132,12 @@ void free_buffers(rpwa_lik_ctx* c)
 	hipFree(c->d_prod);
 	hipFree(c->d_partials);
 	hipFree(c->d_sigma);
+	hipFree(c->d_doneCount);
 	hipHostFree(c->h_prod);
 	hipHostFree(c->h_out);
+	c->d_doneCount = nullptr;
+	c->h_flag = nullptr;
+	c->d_flag = nullptr;
 	c->d_amps = nullptr;
 	c->d_sigma = nullptr;
 	c->rankCap = 0;
@@ -188,8 +194,18 @@ int ensure_rank_buffers(rpwa_lik_ctx* c, uint32_t rank)
 	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
 	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
 	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command)
-	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, nOut * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+	// (one more 8-byte slot for the completion flag)
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, (nOut + 1) * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
 	RPWA_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_out), c->h_out, 0));
+	c->h_flag = reinterpret_cast<unsigned long long*>(c->h_out + nOut);
+	c->d_flag = reinterpret_cast<unsigned long long*>(c->d_out + nOut);
+	*c->h_flag = 0;
+	c->seq = 0;
+	if (!c->d_doneCount) {
+		RPWA_HIP_CHECK(hipMalloc(&c->d_doneCount, sizeof(unsigned int)));
+		RPWA_HIP_CHECK(hipMemset(c->d_doneCount, 0, sizeof(unsigned int)));
+	}
+
 	RPWA_HIP_CHECK(hipMalloc(&c->d_partials, need * sizeof(double)));
 	c->partialCap = need;
 	c->rankCap = r;
@@ -428,7 +444,8 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.partials = c->d_partials;
 	a.nElem = nElem;
 	a.sigma = c->d_sigma;
-	RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+	if (c->timing)
+		RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
 	if (!blocks) {
 		RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[slot], a, true, c->nWg[slot], c->stream));
 	} else {
@@ -442,9 +459,13 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, a, c->nWg[slot], c->stream));
 			}
 	}
-	RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
-	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream));
-	RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+	if (c->timing)
+		RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
+	const unsigned long long seq = ++c->seq;
+	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream,
+	                                   c->d_doneCount, c->d_flag, seq));
+	if (c->timing)
+		RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
 	c->pending = true;
 	c->evaluated[slot] = true;
 	c->pendingRank = rank;
@@ -460,9 +481,30 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 	}
 	c->pending = false;
 	RPWA_HIP_CHECK(hipSetDevice(c->device));
-	RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
+	// The reduction's last block stores the evaluation's sequence number into the
+	// mapped flag after the results: watching it avoids the wake-up latency of a
+	// blocking synchronisation.  A flag that does not arrive within a second (a
+	// failed launch, a long kernel) falls back to the stream synchronisation,
+	// which also reports errors.
+	bool done = false;
+	if (!c->timing) {
+		const volatile unsigned long long* flag = c->h_flag;
+		const auto t0 = std::chrono::steady_clock::now();
+		for (uint32_t spin = 0;; ++spin) {
+			if (*flag == c->seq) {
+				done = true;
+				break;
+			}
+			if ((spin & 1023u) == 1023u && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1))
+				break;
+			__builtin_ia32_pause();
+		}
+		std::atomic_thread_fence(std::memory_order_acquire);
+	}
+	if (!done)
+		RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
 	float ms = 0.f, ms1 = 0.f;
-	if (hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
+	if (c->timing && hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
 	    hipEventElapsedTime(&ms1, c->evMid, c->ev1) == hipSuccess) {
 		c->eventTime += 1e-3 * ms;
 		c->reduceTime += 1e-3 * ms1;
@@ -547,6 +589,8 @@ extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
 		e = hipDeviceGetAttribute(&c->nCu, hipDeviceAttributeMultiprocessorCount, device);
 	if (const char* env = std::getenv("RPWA_AUTOTUNE"))
 		c->autotune = std::string(env) != "0";
+	if (const char* env = std::getenv("RPWA_KERNEL_TIMING"))
+		c->timing = std::string(env) != "0";
 	if (e != hipSuccess) {
 		set_error(std::string("rpwa_lik_create: ") + hipGetErrorString(e));
 		if (c->stream)
@@ -856,6 +900,19 @@ extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile
 	}
 	return RPWA_OK;
 	RPWA_GUARD_END
+}
+
+extern "C" int rpwa_lik_set_timing(rpwa_lik_ctx* ctx, int enable)
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_set_timing"))
+		return st;
+	if (ctx->pending) {
+		set_error("rpwa_lik_set_timing: an evaluation is pending");
+		return RPWA_ERR_STATE;
+	}
+	ctx->timing = enable != 0;
+	return RPWA_OK;
 }
 
 extern "C" int rpwa_lik_set_autotune(rpwa_lik_ctx* ctx, int enable)

@@ -112,8 +112,10 @@ bool stage_t_inline(const AmpLayout& lay, int rank);
 int stage_rank_cap(int rank);
 hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s);
 int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu);
+// doneCount / doneFlag may be null (no completion flag)
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
-                               double* out, hipStream_t s);
+                               double* out, hipStream_t s, unsigned int* doneCount = nullptr,
+                               unsigned long long* doneFlag = nullptr, unsigned long long seq = 0);
 // weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T
 int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
              const double* h_x, const double* h_w, std::vector<double>& full, double* wtot);
@@ -149,7 +151,12 @@ struct rpwa_lik_ctx {
 	double* d_partials = nullptr;  // [nWg][partial_elems]
 	double* d_out = nullptr;       // device view of h_out: the reduce kernel writes the results there
 	double* h_prod = nullptr;      // pinned staging
-	double* h_out = nullptr;       // reduced results (pinned, coherent, mapped into the device)
+	double* h_out = nullptr;       // reduced results (pinned, coherent, mapped into the device), then the flag
+	unsigned long long* h_flag = nullptr;  // completion flag after the results (host view) ...
+	unsigned long long* d_flag = nullptr;  // ... and its device view
+	unsigned int* d_doneCount = nullptr;   // blocks of the reduction done (reset by the last one)
+	unsigned long long seq = 0;            // evaluations submitted
+	bool timing = false;                   // per-evaluation kernel timing with events (rpwa_lik_set_timing)
 	size_t partialCap = 0;
 	// pending asynchronous evaluation (rpwa_lik_submit / rpwa_lik_collect)
 	bool pending = false;

@@ -55,9 +55,13 @@ constexpr int kRedElems = 16;
 constexpr int kRedChunks = 64;
 constexpr int kRedBatch = 8;
 
+// Completion: after its results every block adds one to *doneCount; the last
+// block resets the counter and stores seq into *doneFlag (host-visible memory),
+// so a host can wait for the results by watching the flag instead of a
+// blocking stream synchronisation.
 __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
     const double* __restrict__ partials, int nWg, uint32_t nElem, double flat,
-    double* __restrict__ out)
+    double* __restrict__ out, unsigned int* doneCount, unsigned long long* doneFlag, unsigned long long seq)
 {
 	__shared__ double sh[kRedChunks][kRedElems];
 	__shared__ double shc[kRedChunks][kRedElems];
@@ -128,6 +132,18 @@ __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
 			out[1] = flat * t;
 		else
 			out[elem - 1] = t;
+		__threadfence_system();  // this thread's results are visible to the host
+	}
+	if (doneFlag) {
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			const unsigned int prev = __hip_atomic_fetch_add(doneCount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+			if (prev == gridDim.x - 1) {
+				__hip_atomic_store(doneCount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				__threadfence_system();
+				__hip_atomic_store(doneFlag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+			}
+		}
 	}
 }
 
@@ -225,11 +241,12 @@ int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, 
 }
 
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
-                               double* out, hipStream_t s)
+                               double* out, hipStream_t s, unsigned int* doneCount, unsigned long long* doneFlag,
+                               unsigned long long seq)
 {
 	const int nb = (int)((nElem + kRedElems - 1) / kRedElems);
 	hipLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0, s, partials,
-	                   nWg, nElem, flat, out);
+	                   nWg, nElem, flat, out, doneCount, doneFlag, seq);
 	return hipGetLastError();
 }
 

@@ -387,6 +387,7 @@ class pwaLikelihood:
 
     def finishInit(self):
         _call("rpwa_pwalik_finish_init", self.h)
+        self._nPars = self.nmbPars()  # fixed from here on: evaluations check their input against it
 
     # accessors -----------------------------------------------------------
     def nmbPars(self):
@@ -437,7 +438,8 @@ class pwaLikelihood:
     # evaluation ------------------------------------------------------------
     def _par(self, par):
         p = np.ascontiguousarray(par, dtype=np.float64)
-        assert p.shape == (self.nmbPars(),)
+        n = getattr(self, "_nPars", None)
+        assert p.shape == ((n if n is not None else self.nmbPars()),)
         return p
 
     def DoEval(self, par):
