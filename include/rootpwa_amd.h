@@ -123,10 +123,12 @@ int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProd
 
 /* Accumulated time spent in this context's kernels [s]
  * (likelihoodInterface::kernelTime / resetKernelTime, likelihoodInterface.cuh:124-125),
- * measured with HIP events around every evaluation while timing is on
- * (rpwa_lik_set_timing, or RPWA_KERNEL_TIMING=1 in the environment; off by
- * default: the events cost host time on every evaluation, and with timing off
- * an evaluation's completion is read from a flag the reduction writes). */
+ * measured with HIP events bound to the kernel launches of every evaluation
+ * while timing is on (the sum of the event kernels' and the reduction's own
+ * device time, without launch gaps; rpwa_lik_set_timing, or
+ * RPWA_KERNEL_TIMING=1 in the environment; off by default: the events cost host
+ * time on every evaluation, and with timing off an evaluation's completion is
+ * read from a flag the reduction writes). */
 int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds);
 int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx);
 /* Kernel timing on (enable != 0) or off for the following evaluations. */

@@ -331,10 +331,9 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			for (int it = 0; it < reps + 1; ++it) {
 				// the reduction of the workgroup rows is timed with the kernel: its cost
 				// grows with the grid, which differs between the variants
-				RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-				RPWA_HIP_CHECK(launch_event_kernel(c->lay, ts, a, true, c->nWg[rank], c->stream));
-				RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, 0.1, c->d_out, c->stream));
-				RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+				RPWA_HIP_CHECK(launch_event_kernel(c->lay, ts, a, true, c->nWg[rank], c->stream, c->ev0, nullptr));
+				RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, 0.1, c->d_out, c->stream,
+				                                   nullptr, nullptr, 0, nullptr, c->ev1));
 				RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
 				float t = 0.f;
 				RPWA_HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
@@ -444,10 +443,11 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.partials = c->d_partials;
 	a.nElem = nElem;
 	a.sigma = c->d_sigma;
-	if (c->timing)
-		RPWA_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+	// timing events are bound to the launches: the first event launch starts ev0,
+	// the last one stops evMid, the reduction runs between evRed and ev1
+	hipEvent_t kStart = c->timing ? c->ev0 : nullptr, kStop = c->timing ? c->evMid : nullptr;
 	if (!blocks) {
-		RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[slot], a, true, c->nWg[slot], c->stream));
+		RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[slot], a, true, c->nWg[slot], c->stream, kStart, kStop));
 	} else {
 		const size_t tStride = (size_t)c->lay.rowLen * kMaxRank;
 		for (int pass = 0; pass < 2; ++pass)
@@ -456,16 +456,15 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 				a.accumulate = b > 0 ? 1u : 0u;
 				a.rank0 = b * kMaxRank;
 				a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps + b * tStride;
-				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, a, c->nWg[slot], c->stream));
+				const bool first = pass == 0 && b == 0, last = pass == 1 && b + 1 == nBlocks;
+				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, a, c->nWg[slot], c->stream,
+				                                   first ? kStart : nullptr, last ? kStop : nullptr));
 			}
 	}
-	if (c->timing)
-		RPWA_HIP_CHECK(hipEventRecord(c->evMid, c->stream));
 	const unsigned long long seq = ++c->seq;
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream,
-	                                   c->d_doneCount, c->d_flag, seq));
-	if (c->timing)
-		RPWA_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+	                                   c->d_doneCount, c->d_flag, seq, c->timing ? c->evRed : nullptr,
+	                                   c->timing ? c->ev1 : nullptr));
 	c->pending = true;
 	c->evaluated[slot] = true;
 	c->pendingRank = rank;
@@ -505,10 +504,10 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 		RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
 	float ms = 0.f, ms1 = 0.f;
 	if (c->timing && hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
-	    hipEventElapsedTime(&ms1, c->evMid, c->ev1) == hipSuccess) {
+	    hipEventElapsedTime(&ms1, c->evRed, c->ev1) == hipSuccess) {
 		c->eventTime += 1e-3 * ms;
 		c->reduceTime += 1e-3 * ms1;
-		c->kernelTime += 1e-3 * (ms + ms1);
+		c->kernelTime += 1e-3 * ((double)ms + (double)ms1);
 		++c->nEvals;
 	}
 	if (lnl)
@@ -586,6 +585,8 @@ extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
 	if (e == hipSuccess)
 		e = hipEventCreate(&c->evMid);
 	if (e == hipSuccess)
+		e = hipEventCreate(&c->evRed);
+	if (e == hipSuccess)
 		e = hipDeviceGetAttribute(&c->nCu, hipDeviceAttributeMultiprocessorCount, device);
 	if (const char* env = std::getenv("RPWA_AUTOTUNE"))
 		c->autotune = std::string(env) != "0";
@@ -593,6 +594,9 @@ extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
 		c->timing = std::string(env) != "0";
 	if (e != hipSuccess) {
 		set_error(std::string("rpwa_lik_create: ") + hipGetErrorString(e));
+		for (hipEvent_t ev : {c->ev0, c->ev1, c->evMid, c->evRed})
+			if (ev)
+				hipEventDestroy(ev);
 		if (c->stream)
 			hipStreamDestroy(c->stream);
 		delete c;
@@ -624,6 +628,7 @@ extern "C" int rpwa_lik_release(rpwa_lik_ctx* ctx)
 		hipEventDestroy(ctx->ev0);
 		hipEventDestroy(ctx->ev1);
 		hipEventDestroy(ctx->evMid);
+		hipEventDestroy(ctx->evRed);
 		hipStreamDestroy(ctx->stream);
 		delete ctx;
 	}

@@ -1,6 +1,7 @@
 // internal.h -- shared declarations of the gfx950 backend (not installed).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -102,20 +103,25 @@ struct EventArgs {
 // sum of weights -2/sigma, then [rank][2][wmax] complex derivatives.
 inline uint32_t partial_elems(int rank, uint32_t wmax) { return 3u + 2u * (uint32_t)rank * 2u * wmax; }
 
-// launchers (lik_kernels.hip)
+// launchers (lik_kernels.hip).  A non-null start / stop event is bound to the
+// launch itself (hipExtLaunchKernelGGL): it takes the kernel's own start / end
+// time on the device, not the time the host queued it.
 hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
-                               bool grad, int nWg, hipStream_t s);
+                               bool grad, int nWg, hipStream_t s, hipEvent_t start = nullptr,
+                               hipEvent_t stop = nullptr);
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
 bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p);
 // true if T of every row column fits into the staged kernel's arguments (no upload needed)
 bool stage_t_inline(const AmpLayout& lay, int rank);
 int stage_rank_cap(int rank);
-hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s);
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s,
+                               hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu);
 // doneCount / doneFlag may be null (no completion flag)
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s, unsigned int* doneCount = nullptr,
-                               unsigned long long* doneFlag = nullptr, unsigned long long seq = 0);
+                               unsigned long long* doneFlag = nullptr, unsigned long long seq = 0,
+                               hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T
 int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
              const double* h_x, const double* h_w, std::vector<double>& full, double* wtot);
@@ -132,7 +138,8 @@ struct rpwa_lik_ctx {
 	int device = 0;
 	int precision = RPWA_PREC_FP64;
 	hipStream_t stream = nullptr;
-	hipEvent_t ev0 = nullptr, evMid = nullptr, ev1 = nullptr;
+	// kernel timing: ev0 / evMid bracket the event kernel(s), evRed / ev1 the reduction
+	hipEvent_t ev0 = nullptr, evMid = nullptr, evRed = nullptr, ev1 = nullptr;
 	rpwa_amd::AmpLayout lay;
 	// tile and persistent grid per rank (index = rank); fixed before the first
 	// evaluation of that rank, so every result of a rank comes from one tile

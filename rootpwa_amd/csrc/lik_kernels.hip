@@ -218,12 +218,15 @@ size_t lds_bytes(const AmpLayout& lay, const TileShape& ts, int rank)
 }  // namespace
 
 hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const EventArgs& a,
-                               bool /*grad*/, int nWg, hipStream_t s)
+                               bool /*grad*/, int nWg, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
 	if (ts.events > 0)
-		return launch_stage_kernel(lay, ts.events, a, nWg, s);
+		return launch_stage_kernel(lay, ts.events, a, nWg, s, start, stop);
 	EventKernel k = pick_kernel(lay, ts, a.rank);
-	hipLaunchKernelGGL(k, dim3(nWg), dim3(kWgThreads), lds_bytes(lay, ts, a.rank), s, a);
+	if (start || stop)
+		hipExtLaunchKernelGGL(k, dim3(nWg), dim3(kWgThreads), (uint32_t)lds_bytes(lay, ts, a.rank), s, start, stop, 0u, a);
+	else
+		hipLaunchKernelGGL(k, dim3(nWg), dim3(kWgThreads), lds_bytes(lay, ts, a.rank), s, a);
 	return hipGetLastError();
 }
 
@@ -242,11 +245,15 @@ int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, 
 
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s, unsigned int* doneCount, unsigned long long* doneFlag,
-                               unsigned long long seq)
+                               unsigned long long seq, hipEvent_t start, hipEvent_t stop)
 {
 	const int nb = (int)((nElem + kRedElems - 1) / kRedElems);
-	hipLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0, s, partials,
-	                   nWg, nElem, flat, out, doneCount, doneFlag, seq);
+	if (start || stop)
+		hipExtLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0u, s, start, stop, 0u,
+		                      partials, nWg, nElem, flat, out, doneCount, doneFlag, seq);
+	else
+		hipLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0, s, partials,
+		                   nWg, nElem, flat, out, doneCount, doneFlag, seq);
 	return hipGetLastError();
 }
 

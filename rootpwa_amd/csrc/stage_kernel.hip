@@ -545,7 +545,8 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 	return p.ldsBytes <= 160u * 1024u;
 }
 
-hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& ea, int nWg, hipStream_t s)
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& ea, int nWg, hipStream_t s,
+                               hipEvent_t start, hipEvent_t stop)
 {
 	StagePlan p;
 	if (!stage_plan(lay, events, ea.rank, p))
@@ -591,7 +592,10 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 		if (e != hipSuccess)
 			return e;
 	}
-	hipLaunchKernelGGL(k, dim3(nWg), dim3(p.threads), p.ldsBytes, s, a);
+	if (start || stop)
+		hipExtLaunchKernelGGL(k, dim3(nWg), dim3(p.threads), (uint32_t)p.ldsBytes, s, start, stop, 0u, a);
+	else
+		hipLaunchKernelGGL(k, dim3(nWg), dim3(p.threads), p.ldsBytes, s, a);
 	return hipGetLastError();
 }
 

@@ -165,3 +165,35 @@ def test_generator_matches_cpu_restatement(gpu):
     assert pool == pool_c
     assert abs(smax - smax_c) <= 1e-12 * smax_c
     assert H.rel_err(out.download((300, 7), np.complex128), data) <= 1e-13
+
+
+@pytest.mark.parametrize("rank,prec", [(2, 0), (2, 1), (6, 1)], ids=["r2_fp64", "r2_mixed", "r6_mixed_blocks"])
+def test_kernel_timing_leaves_results_unchanged(gpu, rank, prec):
+    """Timing events are bound to the kernel launches: results with timing on
+    are bitwise those with it off (the completion flag path), and the split
+    times cover every evaluation, the reduction much shorter than the event
+    kernels at 50k events x 8+40 waves."""
+    nW = (8, 40)
+    rng = np.random.default_rng(77 + rank)
+    N = 50_000
+    amps = (rng.normal(size=(N, sum(nW))) + 1j * rng.normal(size=(N, sum(nW))))
+    print("timing rank", rank, "prec", prec, flush=True)
+    ctx = gpu.LikelihoodContext(0, prec)
+    ctx.load(amps, nW)
+    prod = H.random_prod(rng, rank, nW, 0.7)
+    ctx.set_timing(False)
+    ref = ctx.fdf(prod, 0.1)
+    ctx.reset_kernel_time()
+    ctx.set_timing(True)
+    for _ in range(4):
+        f, d, df = ctx.fdf(prod, 0.1)
+        assert f == ref[0] and np.array_equal(d, ref[1]) and df == ref[2]
+    ev, red, n = ctx.kernel_times()
+    assert n == 4
+    assert ev > 0 and red > 0 and red < ev
+    assert abs(ctx.kernel_time() - (ev + red)) <= 1e-12 * (ev + red)
+    ctx.set_timing(False)
+    f, d, df = ctx.fdf(prod, 0.1)
+    assert f == ref[0] and np.array_equal(d, ref[1])
+    assert ctx.kernel_times()[2] == 4
+    ctx.close()
