@@ -393,6 +393,10 @@ def main():
                  "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs,
                  "traffic": traffic.get("fp64")},
         "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
+        # the same FdF loop with kernel timing off (the library's default: no events,
+        # completion read from the reduction's flag); value keeps the timed loop above
+        "without_kernel_events": {"mixed_evals_per_s": r_mix["rate_without_events"],
+                                  "fp64_evals_per_s": r_64["rate_without_events"]},
         "integral": {"ms": 1e3 * t_int, "tflops_algorithmic": int_flops / t_int / 1e12,
                      "frac_fp64_mfma": int_flops / t_int / 1e12 / FP64_MFMA_PEAK_TFS, "mc_events": M},
         "bandwidth_probe": {"read_GBs": read_gbs, "copy_GBs": copy_gbs, "bytes": 4 << 30},

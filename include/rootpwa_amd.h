@@ -154,7 +154,11 @@ int rpwa_lik_info(rpwa_lik_ctx* ctx, uint64_t info[9]);
  * lanes per event 1 (LDS-staged: tiles of 64, 32 or 16 events, each timed,
  * pass through LDS, one lane contracts one event, production amplitudes come
  * through the scalar unit; rpwa_lik_set_tile(ctx, 1) takes the largest tile
- * that fits).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
+ * that fits), and for complex64 rows of at most 128 waves the LDS-staged kernel
+ * with its gradient phase on FP32 MFMA (tiles of 64 and 32 events; FP32
+ * products of the amplitudes with u = -2S/sigma rounded to FP32, summed per
+ * tile on the matrix cores and added in FP64; S, sigma and the log sum stay
+ * FP64).  rpwa_lik_set_autotune(ctx, 0) (or RPWA_AUTOTUNE=0 in the
  * environment) keeps the modelled default instead; rpwa_lik_set_tile fixes the
  * variant of every rank up to 4 (RPWA_ERR_UNSUPPORTED if the row does not fit it);
  * ranks above 4 always run the LDS-staged kernel (the largest tile that fits, or
@@ -166,6 +170,15 @@ int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
  * No counterpart in the reference: a tuning and testing control like
  * rpwa_lik_set_tile, for the tile sizes the autotuner chooses between. */
 int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile);
+/* As rpwa_lik_set_stage_tile, choosing also the gradient phase: on FP32 MFMA
+ * (gradientOnMfma != 0; complex64 amplitudes, rows of at most 128 waves, 64 or
+ * 32 events per tile) or FP64 VALU (0).  No counterpart in the reference. */
+int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gradientOnMfma);
+/* The event-kernel variant of a rank: variant[0] lanes per event (1: LDS-staged),
+ * [1] slots per lane (events per tile for the staged kernel), [2] 1 if the
+ * staged kernel's gradient phase runs on MFMA, [3] workgroups.  No counterpart
+ * in the reference. */
+int rpwa_lik_kernel_variant(rpwa_lik_ctx* ctx, uint32_t rank, uint32_t variant[4]);
 
 /* Stream on which the context launches its kernels (for event timing). */
 int rpwa_lik_stream(rpwa_lik_ctx* ctx, void** hipStream);

@@ -58,6 +58,8 @@ PROTOTYPES = {
     "rpwa_lik_info": (_i, [_vp, _u64p]),
     "rpwa_lik_set_tile": (_i, [_vp, _u32]),
     "rpwa_lik_set_stage_tile": (_i, [_vp, _u32]),
+    "rpwa_lik_set_stage_variant": (_i, [_vp, _u32, _i]),
+    "rpwa_lik_kernel_variant": (_i, [_vp, _u32, _u32p]),
     "rpwa_lik_set_autotune": (_i, [_vp, _i]),
     "rpwa_lik_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
     "rpwa_dev_alloc": (_i, [_i, _u64, ctypes.POINTER(_vp)]),
@@ -249,6 +251,14 @@ class LikelihoodContext:
 
     def set_stage_tile(self, events):
         call("rpwa_lik_set_stage_tile", self.h, int(events))
+
+    def set_stage_variant(self, events, gradient_on_mfma):
+        call("rpwa_lik_set_stage_variant", self.h, int(events), int(bool(gradient_on_mfma)))
+
+    def kernel_variant(self, rank):
+        out = (ctypes.c_uint32 * 4)()
+        call("rpwa_lik_kernel_variant", self.h, int(rank), out)
+        return dict(zip(("lanes", "slots", "gradient_mfma", "workgroups"), list(out)))
 
     def set_autotune(self, enable):
         call("rpwa_lik_set_autotune", self.h, int(bool(enable)))

@@ -65,16 +65,17 @@ bool make_tile(const AmpLayout& lay, int lanes, TileShape& ts)
 }
 
 // Tile of the LDS-staged kernel: the largest feasible number of events per
-// tile, at most maxEvents.
-bool make_stage_tile(const AmpLayout& lay, int rank, TileShape& ts, int maxEvents = 64)
+// tile, at most maxEvents; gm = 1 asks for the gradient phase on FP32 MFMA.
+bool make_stage_tile(const AmpLayout& lay, int rank, TileShape& ts, int maxEvents = 64, int gm = 0)
 {
 	for (int e : {64, 32, 16}) {
 		StagePlan p;
-		if (e <= maxEvents && stage_plan(lay, e, rank, p)) {
+		if (e <= maxEvents && stage_plan(lay, e, rank, p, gm)) {
 			ts = TileShape();
 			ts.lanes = 1;
 			ts.slots = e;
 			ts.events = e;
+			ts.gm = gm;
 			return true;
 		}
 	}
@@ -316,10 +317,14 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	// register tiles of 8..64 lanes per event, then the LDS-staged kernel with
 	// tiles of 64, 32 and 16 events (the smaller tiles give a short event list
 	// more tiles per workgroup)
-	for (int cand : {8, 16, 32, 64, -64, -32, -16}) {
+	// (and, for complex64 rows of at most 64 units, the staged kernel with the
+	// gradient phase on FP32 MFMA, tiles of 64 and 32 events)
+	const int cands[][2] = {{8, 0}, {16, 0}, {32, 0}, {64, 0}, {-64, 0}, {-32, 0}, {-16, 0}, {-64, 1}, {-32, 1}};
+	for (const auto& cg : cands) {
+			const int cand = cg[0], gm = cg[1];
 			TileShape ts;
 			if (cand > 0 ? !make_tile(c->lay, cand, ts)
-			             : (!make_stage_tile(c->lay, rank, ts, -cand) || ts.events != -cand))
+			             : (!make_stage_tile(c->lay, rank, ts, -cand, gm) || ts.events != -cand || ts.gm != gm))
 				continue;
 			c->tile[rank] = ts;
 			if (int st = configure_grid(c, rank))
@@ -344,8 +349,8 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			const double med = ms[ms.size() / 2];
 			if (std::getenv("RPWA_VERBOSE"))
 				std::fprintf(stderr, "[rpwa autotune] precision %d rank %d lanes %d slots %d (%d+%d lanes) "
-				             "events/tile %d workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots,
-				             ts.lanes0, ts.lanes1, ts.events, c->nWg[rank], med);
+				             "events/tile %d%s workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots,
+				             ts.lanes0, ts.lanes1, ts.events, ts.gm ? " (gradient on MFMA)" : "", c->nWg[rank], med);
 			if (med < bestT) {
 				bestT = med;
 				best = ts;
@@ -457,7 +462,7 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 				a.rank0 = b * kMaxRank;
 				a.tpad = reinterpret_cast<const double2*>(c->d_prod) + nmbProdAmps + b * tStride;
 				const bool first = pass == 0 && b == 0, last = pass == 1 && b + 1 == nBlocks;
-				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, a, c->nWg[slot], c->stream,
+				RPWA_HIP_CHECK(launch_stage_kernel(c->lay, c->tile[slot].events, c->tile[slot].gm, a, c->nWg[slot], c->stream,
 				                                   first ? kStart : nullptr, last ? kStop : nullptr));
 			}
 	}
@@ -873,31 +878,34 @@ extern "C" int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent)
 	RPWA_GUARD_END
 }
 
-extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile)
+namespace {
+int set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gm, const char* fn)
 {
 	clear_error();
-	if (int st = check_ctx(ctx, "rpwa_lik_set_stage_tile"))
+	if (int st = check_ctx(ctx, fn))
 		return st;
 	if (!ctx->loaded) {
-		set_error("rpwa_lik_set_stage_tile: decay amplitudes not loaded");
+		set_error(std::string(fn) + ": decay amplitudes not loaded");
 		return RPWA_ERR_STATE;
 	}
 	const int e = (int)eventsPerTile;
 	if (e != 16 && e != 32 && e != 64) {
-		set_error("rpwa_lik_set_stage_tile: events per tile must be 16, 32 or 64");
+		set_error(std::string(fn) + ": events per tile must be 16, 32 or 64");
 		return RPWA_ERR_INVALID;
 	}
 	TileShape ts;
 	for (int r = 1; r <= kMaxRank; ++r)
-		if (!make_stage_tile(ctx->lay, r, ts, e) || ts.events != e) {
-			set_error("rpwa_lik_set_stage_tile: a tile of " + std::to_string(e) +
-			          " amplitude rows does not fit the LDS-staged kernel");
+		if (!make_stage_tile(ctx->lay, r, ts, e, gm) || ts.events != e || ts.gm != gm) {
+			set_error(std::string(fn) + ": a tile of " + std::to_string(e) + " amplitude rows does not fit the " +
+			          (gm ? "LDS-staged kernel with the gradient on MFMA (complex64 rows of at most 128 waves, "
+			                "tiles of 64 or 32 events)"
+			              : "LDS-staged kernel"));
 			return RPWA_ERR_UNSUPPORTED;
 		}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
 	for (int r = 1; r <= kHighRankSlot; ++r) {
-		make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts, e);
+		make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts, e, gm);
 		ctx->tile[r] = ts;
 		ctx->tileFixed[r] = true;
 		if (int st = configure_grid(ctx, r))
@@ -905,6 +913,34 @@ extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile
 	}
 	return RPWA_OK;
 	RPWA_GUARD_END
+}
+}  // namespace
+
+extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile)
+{
+	return set_stage_variant(ctx, eventsPerTile, 0, "rpwa_lik_set_stage_tile");
+}
+
+extern "C" int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gradientOnMfma)
+{
+	return set_stage_variant(ctx, eventsPerTile, gradientOnMfma ? 1 : 0, "rpwa_lik_set_stage_variant");
+}
+
+extern "C" int rpwa_lik_kernel_variant(rpwa_lik_ctx* ctx, uint32_t rank, uint32_t variant[4])
+{
+	clear_error();
+	if (int st = check_ctx(ctx, "rpwa_lik_kernel_variant"))
+		return st;
+	if (!variant || rank == 0) {
+		set_error("rpwa_lik_kernel_variant: null output or rank 0");
+		return RPWA_ERR_INVALID;
+	}
+	const int slot = rank_slot(rank);
+	variant[0] = (uint32_t)ctx->tile[slot].lanes;
+	variant[1] = (uint32_t)ctx->tile[slot].slots;
+	variant[2] = (uint32_t)ctx->tile[slot].gm;
+	variant[3] = (uint32_t)ctx->nWg[slot];
+	return RPWA_OK;
 }
 
 extern "C" int rpwa_lik_set_timing(rpwa_lik_ctx* ctx, int enable)

@@ -61,11 +61,13 @@ struct TileShape {
 	int lanes0 = 0;
 	int lanes1 = 0;
 	int events = 0;  // > 0: the LDS-staged kernel (one lane per event) with tiles of this many events
+	int gm = 0;      // staged kernel: 1 = the gradient phase on FP32 MFMA (complex64 amplitudes)
 };
 
 // Geometry of the LDS-staged kernel (stage_kernel.hip) for one layout, tile size and rank.
 struct StagePlan {
 	int events = 0;
+	int gm = 0;              // gradient phase on FP32 MFMA
 	int threads = 256;       // workgroup size
 	uint32_t sliceUnits = 0; // units per wave in the S phase
 	int ranks = 1;           // ranks held by the kernel instantiation (1, 2 or 4)
@@ -110,13 +112,13 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
                                bool grad, int nWg, hipStream_t s, hipEvent_t start = nullptr,
                                hipEvent_t stop = nullptr);
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu);
-bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p);
+bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p, int gm = 0);
 // true if T of every row column fits into the staged kernel's arguments (no upload needed)
 bool stage_t_inline(const AmpLayout& lay, int rank);
 int stage_rank_cap(int rank);
-hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& a, int nWg, hipStream_t s,
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, int gm, const EventArgs& a, int nWg, hipStream_t s,
                                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
-int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu);
+int stage_kernel_occupancy(const AmpLayout& lay, int events, int gm, int rank, int* wgPerCu);
 // doneCount / doneFlag may be null (no completion flag)
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
                                double* out, hipStream_t s, unsigned int* doneCount = nullptr,

@@ -221,7 +221,7 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
                                bool /*grad*/, int nWg, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
 	if (ts.events > 0)
-		return launch_stage_kernel(lay, ts.events, a, nWg, s, start, stop);
+		return launch_stage_kernel(lay, ts.events, ts.gm, a, nWg, s, start, stop);
 	EventKernel k = pick_kernel(lay, ts, a.rank);
 	if (start || stop)
 		hipExtLaunchKernelGGL(k, dim3(nWg), dim3(kWgThreads), (uint32_t)lds_bytes(lay, ts, a.rank), s, start, stop, 0u, a);
@@ -233,7 +233,7 @@ hipError_t launch_event_kernel(const AmpLayout& lay, const TileShape& ts, const 
 int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, int* wgPerCu)
 {
 	if (ts.events > 0)
-		return stage_kernel_occupancy(lay, ts.events, rank, wgPerCu);
+		return stage_kernel_occupancy(lay, ts.events, ts.gm, rank, wgPerCu);
 	EventKernel k = pick_kernel(lay, ts, rank);
 	int n = 0;
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kWgThreads, lds_bytes(lay, ts, rank));

@@ -31,6 +31,19 @@
 // Every sum has a fixed order for a fixed grid: results are bitwise
 // reproducible and invariant under T -> -T of a (rank, sector) block.
 //
+// Gradient on the matrix cores (mixed precision, rows of at most 64 units):
+// the G phase can instead run on v_mfma_f32_4x4x1f32 (16 blocks of 4x4x1).
+// Block b is one 16-byte unit of the row (its four floats, the real and
+// imaginary parts of two waves, are the rows i), the columns j are the real
+// and imaginary parts of u for two ranks, K is one event: D[b][i][j] =
+// sum_e x_i(e) u_j(e).  Layout on gfx950: A[b][i] in lane 4b+i, B[b][j] in
+// lane 4b+j, D[b][i][j] in register i of lane 4b+j.  Wave w takes a quarter of
+// the tile's events for all units; u is rounded to FP32 (the amplitudes are
+// FP32 already), the products of a wave's events of one tile are summed in
+// FP32 by the matrix core and added into FP64 sums per tile; at the end the
+// waves' sums are added in order and combined into u conj(x).  S, sigma, the
+// log sum and the weights stay FP64 as in the VALU variant.
+//
 // Ranks above the instantiation's R (at most 4 in registers) run in blocks of
 // R ranks over several launches (STAGE_SIGMA, then STAGE_GRAD per block):
 // sigma_e = sum over the blocks of sum_{r in block} |S_r(e)|^2 is accumulated
@@ -63,6 +76,13 @@ template <int R>
 constexpr int part_stride() { return 2 * R + 1; }
 typedef const __attribute__((address_space(4))) double cdouble;
 
+// MFMA gradient: column sets of four (u_re, u_im per rank) and unit groups of 16
+template <int R>
+constexpr int gm_col_sets() { return (2 * R + 3) / 4; }
+constexpr int kGmUnits = 16;     // units per MFMA (one per block)
+constexpr int kGmMaxGroups = 4;  // rows of at most 64 units
+typedef float gm_f4 __attribute__((ext_vector_type(4)));
+
 // T of one row column through the scalar unit (the index is the same in every lane)
 __device__ __forceinline__ double2 scalar_t(cdouble* tp, uint32_t i)
 {
@@ -86,6 +106,7 @@ struct StageArgs {
 	uint32_t gGroups0;      // G-phase unit groups of the refl -1 sector
 	uint32_t gGroups;       // G-phase unit groups of both sectors (<= 256)
 	uint32_t gSubs;         // event subsets per unit group in the G phase
+	uint32_t gmGroups;      // MFMA gradient: unit groups of 16 (ceil(rowUnits / 16))
 	uint32_t stageBytes;    // (E * rowUnitsP + 1) * 16: the rows and the dump unit
 	uint32_t secOff1, nW0, nW1, wmax;
 	const double2* tpad;    // [rowUnits * P][R]: T of each row column, zero on padding
@@ -173,7 +194,7 @@ __device__ __forceinline__ void s_phase(const StageArgs& a, const f32x4* stage, 
 }
 
 // ---- sigma phase: wave 0, lane = event; S holds wave 0's own partial ----
-template <bool MIXED, int R, int E, int NW>
+template <bool MIXED, int R, int E, int NW, bool GM>
 __device__ __forceinline__ void sigma_phase(const StageArgs& a, const double2* part, double2* uOut, int lane,
                                             uint32_t tile, double2 (&S)[2][R], double& mant, int& expo,
                                             double& wsum)
@@ -227,12 +248,79 @@ __device__ __forceinline__ void sigma_phase(const StageArgs& a, const double2* p
 		w = -2. * r;
 		wsum += w;
 	}
+	if (GM) {
+		// FP32 u by [sector][column][event], columns (re, im) per rank, zero past 2R
+		constexpr int NCOL = 4 * gm_col_sets<R>();
+		float* uf = reinterpret_cast<float*>(uOut);
+#pragma unroll
+		for (int s = 0; s < 2; ++s) {
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				uf[(s * NCOL + 2 * r) * E + lane] = (float)(w * S[s][r].x);
+				uf[(s * NCOL + 2 * r + 1) * E + lane] = (float)(w * S[s][r].y);
+			}
+#pragma unroll
+			for (int c = 2 * R; c < NCOL; ++c)
+				uf[(s * NCOL + c) * E + lane] = 0.f;
+		}
+		return;
+	}
 	double2* p = uOut + (size_t)lane * PS;
 #pragma unroll
 	for (int s = 0; s < 2; ++s)
 #pragma unroll
 		for (int r = 0; r < R; ++r)
 			p[s * R + r] = make_double2(w * S[s][r].x, w * S[s][r].y);
+}
+
+// ---- G phase on the matrix cores (mixed): wave wid, events [wid E/4, (wid+1) E/4),
+// every unit group; aOff / bOff are the lane's byte offset in a staged row and
+// float offset in the u area for each group ----
+template <int R, int E, int NG>
+__device__ __forceinline__ void gm_phase(const StageArgs& a, const f32x4* stage, const float* uf, int wid,
+                                         const uint32_t (&aOff)[NG], const uint32_t (&bOff)[NG],
+                                         double (&GD)[NG][gm_col_sets<R>()][4])
+{
+	constexpr int NCS = gm_col_sets<R>();
+	constexpr int EW = E / 4;
+	if (a.mode == STAGE_SIGMA)
+		return;
+	gm_f4 acc[NG][NCS];
+#pragma unroll
+	for (int g = 0; g < NG; ++g)
+#pragma unroll
+		for (int c = 0; c < NCS; ++c)
+			acc[g][c] = gm_f4{0.f, 0.f, 0.f, 0.f};
+	const char* sb = reinterpret_cast<const char*>(stage);
+	const uint32_t rowBytes = a.rowUnitsP * 16u;
+#pragma unroll
+	for (int q = 0; q < EW; q += 4) {
+		const int e0 = wid * EW + q;
+		gm_f4 bv[NG][NCS];
+#pragma unroll
+		for (int g = 0; g < NG; ++g)
+#pragma unroll
+			for (int c = 0; c < NCS; ++c)
+				bv[g][c] = *reinterpret_cast<const gm_f4*>(uf + bOff[g] + c * 4 * E + e0);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			const char* row = sb + (uint32_t)(e0 + k) * rowBytes;
+#pragma unroll
+			for (int g = 0; g < NG; ++g) {
+				const float av = *reinterpret_cast<const float*>(row + aOff[g]);
+#pragma unroll
+				for (int c = 0; c < NCS; ++c)
+					acc[g][c] = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv[g][c][k], acc[g][c], 0, 0, 0);
+			}
+		}
+	}
+#pragma unroll
+	for (int g = 0; g < NG; ++g)
+#pragma unroll
+		for (int c = 0; c < NCS; ++c)
+#pragma unroll
+			for (int i = 0; i < 4; ++i)
+				GD[g][c][i] += (double)acc[g][c][i];
 }
 
 // ---- G phase: thread = (group of UG units of one sector, event subset) ----
@@ -272,6 +360,90 @@ __device__ __forceinline__ void g_phase(const StageArgs& a, const f32x4* stage, 
 	}
 }
 
+// ---- wave 0 at the end: the lanes' -ln sigma sums and weight sums, combined
+// by a fixed butterfly: at each step the lower lane's value is the left operand
+// in both partners, and the rounding error of each addition (TwoSum) goes to
+// the compensation ----
+template <int E>
+__device__ __forceinline__ void write_log_sums(const StageArgs& a, double* out, int lane, double mant, int expo,
+                                               double wsum)
+{
+	const bool ev = E == 64 || lane < E;
+	double s = ev ? -(log(mant) + (double)expo * 0.69314718055994530942) : 0.;
+	double c = 0., ws = ev ? wsum : 0.;
+#pragma unroll
+	for (int m = 32; m >= 1; m >>= 1) {
+		const double so = __shfl_xor(s, m), co = __shfl_xor(c, m), wo = __shfl_xor(ws, m);
+		const bool upper = (lane & m) != 0;
+		const double x = upper ? so : s, y = upper ? s : so;
+		const double t = x + y;
+		const double yp = t - x;
+		const double err = (x - (t - yp)) + (y - yp);
+		c = (upper ? co + c : c + co) + err;
+		ws = upper ? wo + ws : ws + wo;
+		s = t;
+	}
+	if (lane == 0 && a.rank0 == 0) {
+		out[0] = s;
+		out[1] = c;
+		out[2] = ws;
+	}
+}
+
+// ---- end of the MFMA variant: every wave's FP64 sums D[unit][column][row] go
+// to LDS, then each gradient entry is formed from them, waves in order:
+// u conj(x) = (u_re x_re + u_im x_im) + i (u_im x_re - u_re x_im) ----
+template <int R, int E, int NG, int NT>
+__device__ __forceinline__ void write_partial_row_gm(const StageArgs& a, f32x4* smem, int tid, int lane, int wid,
+                                                     const double (&GD)[NG][gm_col_sets<R>()][4], double mant,
+                                                     int expo, double wsum)
+{
+	constexpr int NCS = gm_col_sets<R>();
+	constexpr int NCOL = 4 * NCS;
+	constexpr int UN = kGmUnits * NG;  // units per wave in the buffer
+	lds_barrier();
+	double* gbuf = reinterpret_cast<double*>(smem);  // [wave][UN][NCOL][4]
+#pragma unroll
+	for (int g = 0; g < NG; ++g)
+#pragma unroll
+		for (int c = 0; c < NCS; ++c) {
+			double* p = gbuf + (((size_t)wid * UN + g * kGmUnits + lane / 4) * NCOL + c * 4 + lane % 4) * 4;
+#pragma unroll
+			for (int i = 0; i < 4; ++i)
+				p[i] = GD[g][c][i];
+		}
+	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
+	if (wid == 0)
+		write_log_sums<E>(a, out, lane, mant, expo, wsum);
+	lds_barrier();
+	const uint32_t nCplx = (a.nElem - 3) / 2;
+	for (uint32_t idx = tid; idx < nCplx; idx += NT) {
+		const uint32_t j = idx % a.wmax;
+		const uint32_t rs = idx / a.wmax;
+		const int s = (int)(rs & 1);
+		const int rg = (int)(rs >> 1);
+		if (rg < (int)a.rank0 || rg >= (int)a.rank0 + R)
+			continue;
+		const int r = rg - (int)a.rank0;
+		double2 v = make_double2(0., 0.);
+		if (j < (s ? a.nW1 : a.nW0)) {
+			const uint32_t c = (s ? a.secOff1 : 0u) + j;
+			const uint32_t unit = c / 2;
+			const uint32_t k = c % 2;
+#pragma unroll
+			for (int w = 0; w < kWaves; ++w) {
+				const double* q = gbuf + ((size_t)w * UN + unit) * NCOL * 4;
+				const double* ur = q + (2 * r) * 4;      // column u_re
+				const double* ui = q + (2 * r + 1) * 4;  // column u_im
+				v.x += ur[2 * k] + ui[2 * k + 1];
+				v.y += ui[2 * k] - ur[2 * k + 1];
+			}
+		}
+		out[3 + 2 * idx] = v.x;
+		out[3 + 2 * idx + 1] = v.y;
+	}
+}
+
 // ---- end: the workgroup's partial row (lnL sum and its compensation, weight
 // sum, gradient entries), in the format of the register-tile kernels ----
 template <bool MIXED, int R, int E, int NT>
@@ -299,31 +471,8 @@ __device__ __forceinline__ void write_partial_row(const StageArgs& a, f32x4* sme
 		}
 	}
 	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
-	if (wid == 0) {
-		// the lanes' -ln sigma sums and weight sums, combined by a fixed butterfly:
-		// at each step the lower lane's value is the left operand in both partners,
-		// and the rounding error of each addition (TwoSum) goes to the compensation
-		const bool ev = E == 64 || lane < E;
-		double s = ev ? -(log(mant) + (double)expo * 0.69314718055994530942) : 0.;
-		double c = 0., ws = ev ? wsum : 0.;
-#pragma unroll
-		for (int m = 32; m >= 1; m >>= 1) {
-			const double so = __shfl_xor(s, m), co = __shfl_xor(c, m), wo = __shfl_xor(ws, m);
-			const bool upper = (lane & m) != 0;
-			const double x = upper ? so : s, y = upper ? s : so;
-			const double t = x + y;
-			const double yp = t - x;
-			const double err = (x - (t - yp)) + (y - yp);
-			c = (upper ? co + c : c + co) + err;
-			ws = upper ? wo + ws : ws + wo;
-			s = t;
-		}
-		if (lane == 0 && a.rank0 == 0) {
-			out[0] = s;
-			out[1] = c;
-			out[2] = ws;
-		}
-	}
+	if (wid == 0)
+		write_log_sums<E>(a, out, lane, mant, expo, wsum);
 	lds_barrier();
 	// element 3 + 2*((r*2 + s)*wmax + j) (+1 for the imaginary part)
 	const uint32_t nCplx = (a.nElem - 3) / 2;
@@ -404,12 +553,17 @@ __device__ __forceinline__ void store_tile(f32x4* stage, const uint32_t (&ldsPos
 }
 
 // The staged kernel: 4 waves, one tile stage; per tile: stage the prefetched
-// tile, S, sigma, G, each phase behind a workgroup barrier.
-template <bool MIXED, int R, int E>
+// tile, S, sigma, G, each phase behind a workgroup barrier.  NG > 0 (mixed
+// precision only): the G phase on the matrix cores with NG unit groups of 16.
+template <bool MIXED, int R, int E, int NG>
 __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 {
 	constexpr int P = MIXED ? 2 : 1;  // complex values per 16-byte unit
 	constexpr int UG = group_units<MIXED, R>();
+	constexpr bool GM = NG > 0;
+	constexpr int NGX = GM ? NG : 1;
+	constexpr int NCS = gm_col_sets<R>();
+	static_assert(!GM || MIXED, "the MFMA gradient needs complex64 amplitudes");
 	extern __shared__ f32x4 smem[];
 	f32x4* stage = smem;  // [E][rowUnitsP] units, then the dump unit
 	double2* part = reinterpret_cast<double2*>(reinterpret_cast<char*>(smem) + a.stageBytes);  // [3][E][PS]
@@ -436,6 +590,25 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 #pragma unroll
 			for (int r = 0; r < R; ++r)
 				G[j][k][r] = make_double2(0., 0.);
+	// MFMA G phase: per unit group the lane's byte offset in a staged row (units
+	// past the row read its last unit; their sums are never used) and its float
+	// offset in the u area (sector of the unit, column lane % 4)
+	double GD[NGX][NCS][4];
+	uint32_t aOff[NGX], bOff[NGX];
+	if constexpr (GM) {
+#pragma unroll
+		for (int g = 0; g < NG; ++g) {
+			const uint32_t unit = (uint32_t)(g * kGmUnits + lane / 4);
+			aOff[g] = min(unit, a.rowUnits - 1) * 16u + (uint32_t)(lane % 4) * 4u;
+			const uint32_t sec = unit >= a.secUnit ? 1u : 0u;
+			bOff[g] = (sec * 4u * NCS + (uint32_t)(lane % 4)) * (uint32_t)E;
+#pragma unroll
+			for (int c = 0; c < NCS; ++c)
+#pragma unroll
+				for (int i = 0; i < 4; ++i)
+					GD[g][c][i] = 0.;
+		}
+	}
 	double mant = 1., wsum = 0.;
 	int expo = 0;
 
@@ -459,40 +632,61 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 			s_phase<MIXED, R, E>(a, stage, part, lane, wid, S);
 			lds_barrier();
 			if (wid == 0)
-				sigma_phase<MIXED, R, E, kWaves>(a, part, part, lane, tile, S, mant, expo, wsum);  // u replaces wave 1's partial
+				sigma_phase<MIXED, R, E, kWaves, GM>(a, part, part, lane, tile, S, mant, expo, wsum);  // u replaces wave 1's partial
 			lds_barrier();
-			g_phase<MIXED, R, E>(a, stage, part, gOn, gU0, gSub, gSec, G);
+			if constexpr (GM)
+				gm_phase<R, E, NGX>(a, stage, reinterpret_cast<const float*>(part), wid, aOff, bOff, GD);
+			else
+				g_phase<MIXED, R, E>(a, stage, part, gOn, gU0, gSub, gSec, G);
 		}
 	}
 	if (a.mode == STAGE_SIGMA)
 		return;  // uniform: the whole workgroup leaves
-	write_partial_row<MIXED, R, E, kThreads>(a, smem, tid, lane, wid, gOn, gU0, gEnd, gSub, G, mant, expo, wsum);
+	if constexpr (GM)
+		write_partial_row_gm<R, E, NGX, kThreads>(a, smem, tid, lane, wid, GD, mant, expo, wsum);
+	else
+		write_partial_row<MIXED, R, E, kThreads>(a, smem, tid, lane, wid, gOn, gU0, gEnd, gSub, G, mant, expo, wsum);
 }
 
 using StageKernel = void (*)(StageArgs);
 
-template <bool MIXED, int E>
+template <bool MIXED, int E, int NG>
 StageKernel pick_r(int rank)
 {
 	if (rank <= 1)
-		return stage_kernel<MIXED, 1, E>;
+		return stage_kernel<MIXED, 1, E, NG>;
 	if (rank <= 2)
-		return stage_kernel<MIXED, 2, E>;
-	return stage_kernel<MIXED, 4, E>;
+		return stage_kernel<MIXED, 2, E, NG>;
+	return stage_kernel<MIXED, 4, E, NG>;
 }
 
 template <bool MIXED>
 StageKernel pick_e(int events, int rank)
 {
 	switch (events) {
-	case 64: return pick_r<MIXED, 64>(rank);
-	case 32: return pick_r<MIXED, 32>(rank);
-	default: return pick_r<MIXED, 16>(rank);
+	case 64: return pick_r<MIXED, 64, 0>(rank);
+	case 32: return pick_r<MIXED, 32, 0>(rank);
+	default: return pick_r<MIXED, 16, 0>(rank);
 	}
 }
 
-StageKernel pick_stage(const AmpLayout& lay, int events, int rank)
+template <int E>
+StageKernel pick_gm(int groups, int rank)
 {
+	switch (groups) {
+	case 1: return pick_r<true, E, 1>(rank);
+	case 2: return pick_r<true, E, 2>(rank);
+	case 3: return pick_r<true, E, 3>(rank);
+	default: return pick_r<true, E, 4>(rank);
+	}
+}
+
+StageKernel pick_stage(const AmpLayout& lay, int events, int gm, int rank)
+{
+	if (gm) {
+		const int groups = (int)((lay.rowLen * 8u / 16u + kGmUnits - 1) / kGmUnits);
+		return events == 64 ? pick_gm<64>(groups, rank) : pick_gm<32>(groups, rank);
+	}
 	return lay.precision == RPWA_PREC_MIXED ? pick_e<true>(events, rank) : pick_e<false>(events, rank);
 }
 
@@ -505,10 +699,14 @@ bool stage_t_inline(const AmpLayout& lay, int rank)
 	return (size_t)lay.rowLen * stage_rank_cap(rank) <= (size_t)kInlineT;
 }
 
-bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
+bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p, int gm)
 {
 	if (events != 16 && events != 32 && events != 64)
 		return false;
+	// the MFMA gradient: complex64 rows of at most 64 units, tiles of 64 or 32 events
+	if (gm && (lay.precision != RPWA_PREC_MIXED || events == 16 || lay.rowLen * 8u > 16u * kGmUnits * kGmMaxGroups))
+		return false;
+	p.gm = gm;
 	const int threads = kThreads;
 	const int waves = kWaves;
 	p.threads = threads;
@@ -542,14 +740,21 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p)
 	const size_t endBytes = (size_t)p.gSubs * p.rowUnits * P * p.ranks * sizeof(double2) +
 	                        (size_t)events * 3 * sizeof(double);
 	p.ldsBytes = std::max((size_t)p.stageBytes + partBytes, endBytes);
+	if (gm) {
+		// the waves' FP64 sums [wave][units][columns][4] at the end
+		const uint32_t groups = (p.rowUnits + kGmUnits - 1) / kGmUnits;
+		const int ncs = p.ranks == 1 ? gm_col_sets<1>() : p.ranks == 2 ? gm_col_sets<2>() : gm_col_sets<4>();
+		const size_t gmBytes = (size_t)waves * groups * kGmUnits * 4 * ncs * 4 * sizeof(double);
+		p.ldsBytes = std::max(p.ldsBytes, gmBytes);
+	}
 	return p.ldsBytes <= 160u * 1024u;
 }
 
-hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs& ea, int nWg, hipStream_t s,
+hipError_t launch_stage_kernel(const AmpLayout& lay, int events, int gm, const EventArgs& ea, int nWg, hipStream_t s,
                                hipEvent_t start, hipEvent_t stop)
 {
 	StagePlan p;
-	if (!stage_plan(lay, events, ea.rank, p))
+	if (!stage_plan(lay, events, ea.rank, p, gm))
 		return hipErrorInvalidValue;
 	StageArgs a;
 	a.amps = reinterpret_cast<const f32x4*>(ea.amps);
@@ -569,6 +774,7 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 	a.gGroups0 = p.gGroups0;
 	a.gGroups = p.gGroups;
 	a.gSubs = p.gSubs;
+	a.gmGroups = (p.rowUnits + kGmUnits - 1) / kGmUnits;
 	a.stageBytes = p.stageBytes;
 	a.secOff1 = lay.secOff[1];
 	a.nW0 = lay.nW[0];
@@ -585,7 +791,7 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 		for (size_t i = 0; i < n; ++i)
 			a.tin[i] = ea.tpadHost[i];
 	}
-	StageKernel k = pick_stage(lay, events, ea.rank);
+	StageKernel k = pick_stage(lay, events, gm, ea.rank);
 	if (p.ldsBytes > 64u * 1024u) {
 		hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
 		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.ldsBytes);
@@ -599,12 +805,12 @@ hipError_t launch_stage_kernel(const AmpLayout& lay, int events, const EventArgs
 	return hipGetLastError();
 }
 
-int stage_kernel_occupancy(const AmpLayout& lay, int events, int rank, int* wgPerCu)
+int stage_kernel_occupancy(const AmpLayout& lay, int events, int gm, int rank, int* wgPerCu)
 {
 	StagePlan p;
-	if (!stage_plan(lay, events, rank, p))
+	if (!stage_plan(lay, events, rank, p, gm))
 		return RPWA_ERR_UNSUPPORTED;
-	StageKernel k = pick_stage(lay, events, rank);
+	StageKernel k = pick_stage(lay, events, gm, rank);
 	if (p.ldsBytes > 64u * 1024u)
 		if (hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
 		                        (int)p.ldsBytes) != hipSuccess)

@@ -93,6 +93,71 @@ def test_every_tile_shape_agrees(gpu, case, prec):
         assert ctx.log_likelihood(p2, 0.1) == f
 
 
+@pytest.mark.parametrize("events", [64, 32])
+def test_stage_gradient_on_mfma(gpu, case, events):
+    """The staged kernel with its gradient phase on FP32 MFMA (complex64 rows of
+    at most 128 waves) against the oracle and against the FP64-VALU gradient
+    of the same tile: the value is computed by the same FP64 code (bitwise
+    equal when the grids agree), the gradient within 1e-6 of its largest entry;
+    sign flip of a (rank, sector) block and the value-only call bitwise."""
+    ds, L, amps = case
+    nW, rank = ds["nW"], ds["rank"]
+    ctx = gpu.LikelihoodContext(0, 1)
+    ctx.load(amps, nW)
+    prod = H.random_prod(np.random.default_rng(31), rank, nW, 0.9)
+    try:
+        ctx.set_stage_variant(events, True)
+    except gpu.RpwaError as e:
+        assert e.code == gpu.RPWA_ERR_UNSUPPORTED and sum(nW) + (nW[0] % 2) + (nW[1] % 2) > 128
+        ctx.close()
+        return
+    print("mfma gradient", nW, "rank", rank, "tile", events, flush=True)
+    assert ctx.kernel_variant(rank)["gradient_mfma"] == 1
+    f, d, df = ctx.fdf(prod, 0.1)
+    wg = ctx.kernel_variant(rank)["workgroups"]
+    f_ref = L.event_loglik(prod, 0.1)
+    d_ref, df_ref = L.event_deriv(prod, 0.1)
+    assert abs(f - f_ref) <= 1e-6 * abs(f_ref)
+    assert H.rel_err(d, d_ref) <= 1e-5
+    assert abs(df - df_ref) <= 1e-5 * abs(df_ref)
+    assert ctx.log_likelihood(prod, 0.1) == f
+    d2, df2 = ctx.log_likelihood_deriv(prod, 0.1)
+    assert np.array_equal(d2, d) and df2 == df
+    p2 = prod.copy()
+    p2[rank - 1, 1] *= -1
+    assert ctx.log_likelihood(p2, 0.1) == f
+    ctx.set_stage_variant(events, False)
+    fv, dv, dfv = ctx.fdf(prod, 0.1)
+    if ctx.kernel_variant(rank)["workgroups"] == wg:
+        assert fv == f and dfv == df
+    assert np.max(np.abs(d - dv)) <= 1e-6 * np.max(np.abs(dv))
+    ctx.close()
+
+
+def test_stage_gradient_on_mfma_rejected(gpu):
+    """FP64 amplitudes, 16-event tiles and rows of more than 128 waves do not
+    take the MFMA gradient."""
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=(100, 20)) + 1j * rng.normal(size=(100, 20))
+    ctx = gpu.LikelihoodContext(0, 0)
+    ctx.load(a, (4, 16))
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(64, True)
+    ctx.close()
+    ctx = gpu.LikelihoodContext(0, 1)
+    ctx.load(a, (4, 16))
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(16, True)
+    ctx.set_stage_variant(32, True)
+    ctx.close()
+    b = rng.normal(size=(100, 130)) + 1j * rng.normal(size=(100, 130))
+    ctx = gpu.LikelihoodContext(0, 1)
+    ctx.load(b, (2, 128))
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(64, True)
+    ctx.close()
+
+
 def test_reshuffled_layout(gpu):
     """The [2][Wmax][N] layout pwaLikelihood::finishInit builds for the GPU."""
     ds = H.dataset((3, 6), 2, 500, 2000, seed=4)

@@ -118,7 +118,8 @@ def test_small_rows_staged_kernel(gpu, nW, nEvents):
 @pytest.mark.parametrize("nEvents", [1, 16, 17, 33, 4097])
 def test_every_staged_tile_size(gpu, nEvents):
     """The LDS-staged kernel with tiles of 64, 32 and 16 events (each a
-    candidate of the autotuner) against the oracle, both precisions."""
+    candidate of the autotuner; with complex64 also the MFMA gradient at 64
+    and 32) against the oracle, both precisions."""
     nW, rank = (8, 20), 2
     ds = H.dataset(nW, rank, nData=nEvents, nMC=1000, seed=900 + nEvents)
     L = H.oracle_likelihood(ds)
@@ -129,9 +130,10 @@ def test_every_staged_tile_size(gpu, nEvents):
     for prec, tol_f, tol_d in ((0, 1e-12, 1e-11), (1, 1e-6, 1e-5)):
         ctx = gpu.LikelihoodContext(0, prec)
         ctx.load(amps, nW)
-        for events in (64, 32, 16):
-            ctx.set_stage_tile(events)
-            print("staged tile", events, "events", nEvents, "precision", prec, flush=True)
+        variants = [(e, False) for e in (64, 32, 16)] + ([(64, True), (32, True)] if prec == 1 else [])
+        for events, gm in variants:
+            ctx.set_stage_variant(events, gm)
+            print("staged tile", events, "mfma gradient", gm, "events", nEvents, "precision", prec, flush=True)
             f, d, df = ctx.fdf(prod, 0.3)
             assert abs(f - f_ref) <= tol_f * max(1.0, abs(f_ref))
             assert np.max(np.abs(d - d_ref)) <= tol_d * np.max(np.abs(d_ref))

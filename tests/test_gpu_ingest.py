@@ -153,16 +153,21 @@ def test_steady_state_every_variant(gpu, long_list, precision):
     ctx = gpu.LikelihoodContext(0, precision)
     ctx.load(amps, nW)
     variants = [("stage", e) for e in (64, 32, 16)] + [("lanes", l) for l in (8, 16, 32, 64)]
+    variants += [("stage_mfma", e) for e in (64, 32)]
     for kind, v in variants:
         try:
-            ctx.set_stage_tile(v) if kind == "stage" else ctx.set_tile(v)
+            if kind == "lanes":
+                ctx.set_tile(v)
+            else:
+                ctx.set_stage_variant(v, kind == "stage_mfma")
         except gpu.RpwaError as e:
             assert e.code == gpu.RPWA_ERR_UNSUPPORTED
+            assert kind != "stage_mfma" or precision == 0
             continue
         f, d, df = ctx.fdf(prod, 0.3)
         info = ctx.info()
         nWg = int(info["workgroups"])
-        perSweep = nWg * (v if kind == "stage" else 256 // v)
+        perSweep = nWg * (v if kind != "lanes" else 256 // v)
         print("steady state precision", precision, kind, v, "workgroups", nWg, "sweeps", N / perSweep, flush=True)
         assert N >= 2 * perSweep
         assert abs(f - f_ref) <= tol_f * max(1.0, abs(f_ref))

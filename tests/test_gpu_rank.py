@@ -45,9 +45,11 @@ def test_high_rank_event_sums(gpu, sample, precision):
         assert ctx.log_likelihood(prod, 0.3) == f
         f2, d2, df2 = ctx.fdf(prod, 0.3)
         assert f2 == f and np.array_equal(d2, d) and df2 == df
-        for events in (64, 32, 16):
-            ctx.set_stage_tile(events)
-            print("rank", rank, "precision", precision, "events", amps.shape[0], "tile", events, flush=True)
+        variants = [(e, False) for e in (64, 32, 16)] + ([(64, True), (32, True)] if precision == 1 else [])
+        for events, gm in variants:
+            ctx.set_stage_variant(events, gm)
+            print("rank", rank, "precision", precision, "events", amps.shape[0], "tile", events, "mfma", gm,
+                  flush=True)
             f, d, df = ctx.fdf(prod, 0.3)
             assert abs(f - f_ref) <= tol_f * max(1.0, abs(f_ref))
             assert np.max(np.abs(d - d_ref)) <= tol_d * np.max(np.abs(d_ref))
