@@ -108,6 +108,12 @@ int rpwa_pwalik_parameter(const rpwa_pwalik* L, uint32_t i, char* name, int name
 int rpwa_pwalik_doeval(const rpwa_pwalik* L, const double* par, double* value);
 int rpwa_pwalik_gradient(const rpwa_pwalik* L, const double* par, double* gradient);
 int rpwa_pwalik_fdf(const rpwa_pwalik* L, const double* par, double* value, double* gradient);
+/* n calls of FdF at par from C++, as a native minimiser makes them; *seconds is
+ * the wall time of the n calls, value/gradient the last result.  No counterpart
+ * in the reference: it measures the native call path without the overhead of a
+ * foreign-function binding. */
+int rpwa_pwalik_time_fdf(const rpwa_pwalik* L, const double* par, uint32_t n, double* value, double* gradient,
+                         double* seconds);
 int rpwa_pwalik_doderivative(const rpwa_pwalik* L, const double* par, uint32_t k, double* value);
 /* nmbPars x nmbPars row-major */
 int rpwa_pwalik_hessian(const rpwa_pwalik* L, const double* par, double* hessian);

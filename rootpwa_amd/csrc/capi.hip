@@ -133,10 +133,8 @@ void free_buffers(rpwa_lik_ctx* c)
 	hipFree(c->d_prod);
 	hipFree(c->d_partials);
 	hipFree(c->d_sigma);
-	hipFree(c->d_doneCount);
 	hipHostFree(c->h_prod);
 	hipHostFree(c->h_out);
-	c->d_doneCount = nullptr;
 	c->h_flag = nullptr;
 	c->d_flag = nullptr;
 	c->d_amps = nullptr;
@@ -194,18 +192,17 @@ int ensure_rank_buffers(rpwa_lik_ctx* c, uint32_t rank)
 	c->partialCap = 0;
 	RPWA_HIP_CHECK(hipMalloc(&c->d_prod, nProd * sizeof(double2)));
 	RPWA_HIP_CHECK(hipHostMalloc(&c->h_prod, nProd * sizeof(double2), hipHostMallocDefault));
-	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command)
-	// (one more 8-byte slot for the completion flag)
-	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, (nOut + 1) * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+	// results land in coherent pinned memory written by the reduce kernel itself (no copy-back command),
+	// followed by one 8-byte completion flag per block of the reduction
+	const size_t nFlags = reduce_blocks((uint32_t)nOut);
+	RPWA_HIP_CHECK(hipHostMalloc(&c->h_out, (nOut + nFlags) * sizeof(double),
+	                             hipHostMallocMapped | hipHostMallocCoherent));
 	RPWA_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_out), c->h_out, 0));
 	c->h_flag = reinterpret_cast<unsigned long long*>(c->h_out + nOut);
 	c->d_flag = reinterpret_cast<unsigned long long*>(c->d_out + nOut);
-	*c->h_flag = 0;
+	for (size_t b = 0; b < nFlags; ++b)
+		c->h_flag[b] = 0;
 	c->seq = 0;
-	if (!c->d_doneCount) {
-		RPWA_HIP_CHECK(hipMalloc(&c->d_doneCount, sizeof(unsigned int)));
-		RPWA_HIP_CHECK(hipMemset(c->d_doneCount, 0, sizeof(unsigned int)));
-	}
 
 	RPWA_HIP_CHECK(hipMalloc(&c->d_partials, need * sizeof(double)));
 	c->partialCap = need;
@@ -338,7 +335,7 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 				// grows with the grid, which differs between the variants
 				RPWA_HIP_CHECK(launch_event_kernel(c->lay, ts, a, true, c->nWg[rank], c->stream, c->ev0, nullptr));
 				RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[rank], nElem, 0.1, c->d_out, c->stream,
-				                                   nullptr, nullptr, 0, nullptr, c->ev1));
+				                                   nullptr, 0, nullptr, c->ev1));
 				RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
 				float t = 0.f;
 				RPWA_HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
@@ -468,8 +465,9 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	}
 	const unsigned long long seq = ++c->seq;
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream,
-	                                   c->d_doneCount, c->d_flag, seq, c->timing ? c->evRed : nullptr,
+	                                   c->d_flag, seq, c->timing ? c->evRed : nullptr,
 	                                   c->timing ? c->ev1 : nullptr));
+	c->pendingBlocks = reduce_blocks(nElem);
 	c->pending = true;
 	c->evaluated[slot] = true;
 	c->pendingRank = rank;
@@ -485,17 +483,20 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 	}
 	c->pending = false;
 	RPWA_HIP_CHECK(hipSetDevice(c->device));
-	// The reduction's last block stores the evaluation's sequence number into the
-	// mapped flag after the results: watching it avoids the wake-up latency of a
-	// blocking synchronisation.  A flag that does not arrive within a second (a
-	// failed launch, a long kernel) falls back to the stream synchronisation,
+	// Every block of the reduction stores the evaluation's sequence number into
+	// its mapped flag after its results: watching them avoids the wake-up latency
+	// of a blocking synchronisation.  Flags that do not arrive within a second (a
+	// failed launch, a long kernel) fall back to the stream synchronisation,
 	// which also reports errors.
 	bool done = false;
 	if (!c->timing) {
 		const volatile unsigned long long* flag = c->h_flag;
 		const auto t0 = std::chrono::steady_clock::now();
+		uint32_t seen = 0;  // blocks 0..seen-1 are done
 		for (uint32_t spin = 0;; ++spin) {
-			if (*flag == c->seq) {
+			while (seen < c->pendingBlocks && flag[seen] == c->seq)
+				++seen;
+			if (seen == c->pendingBlocks) {
 				done = true;
 				break;
 			}

@@ -119,9 +119,10 @@ int stage_rank_cap(int rank);
 hipError_t launch_stage_kernel(const AmpLayout& lay, int events, int gm, const EventArgs& a, int nWg, hipStream_t s,
                                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 int stage_kernel_occupancy(const AmpLayout& lay, int events, int gm, int rank, int* wgPerCu);
-// doneCount / doneFlag may be null (no completion flag)
+// doneFlag (one slot per block of the reduction, reduce_blocks(nElem)) may be null (no completion flags)
+uint32_t reduce_blocks(uint32_t nElem);
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
-                               double* out, hipStream_t s, unsigned int* doneCount = nullptr,
+                               double* out, hipStream_t s,
                                unsigned long long* doneFlag = nullptr, unsigned long long seq = 0,
                                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // weighted real rank-k update (integral.hip): full D x D matrix of sum_e wt_e x_e x_e^T
@@ -161,9 +162,9 @@ struct rpwa_lik_ctx {
 	double* d_out = nullptr;       // device view of h_out: the reduce kernel writes the results there
 	double* h_prod = nullptr;      // pinned staging
 	double* h_out = nullptr;       // reduced results (pinned, coherent, mapped into the device), then the flag
-	unsigned long long* h_flag = nullptr;  // completion flag after the results (host view) ...
-	unsigned long long* d_flag = nullptr;  // ... and its device view
-	unsigned int* d_doneCount = nullptr;   // blocks of the reduction done (reset by the last one)
+	unsigned long long* h_flag = nullptr;  // completion flags of the reduction's blocks after the results (host view) ...
+	unsigned long long* d_flag = nullptr;  // ... and their device view
+	uint32_t pendingBlocks = 0;            // blocks of the pending evaluation's reduction
 	unsigned long long seq = 0;            // evaluations submitted
 	bool timing = false;                   // per-evaluation kernel timing with events (rpwa_lik_set_timing)
 	size_t partialCap = 0;

@@ -55,13 +55,13 @@ constexpr int kRedElems = 16;
 constexpr int kRedChunks = 64;
 constexpr int kRedBatch = 8;
 
-// Completion: after its results every block adds one to *doneCount; the last
-// block resets the counter and stores seq into *doneFlag (host-visible memory),
-// so a host can wait for the results by watching the flag instead of a
-// blocking stream synchronisation.
+// Completion: after its results (each written with a system-scope fence) every
+// block b stores seq into doneFlag[b] (host-visible memory), so a host can wait
+// for the results by watching the flags of all blocks instead of a blocking
+// stream synchronisation; no block waits for another.
 __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
     const double* __restrict__ partials, int nWg, uint32_t nElem, double flat,
-    double* __restrict__ out, unsigned int* doneCount, unsigned long long* doneFlag, unsigned long long seq)
+    double* __restrict__ out, unsigned long long* doneFlag, unsigned long long seq)
 {
 	__shared__ double sh[kRedChunks][kRedElems];
 	__shared__ double shc[kRedChunks][kRedElems];
@@ -136,14 +136,8 @@ __global__ __launch_bounds__(kRedElems* kRedChunks) void event_reduce_kernel(
 	}
 	if (doneFlag) {
 		__syncthreads();
-		if (threadIdx.x == 0) {
-			const unsigned int prev = __hip_atomic_fetch_add(doneCount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-			if (prev == gridDim.x - 1) {
-				__hip_atomic_store(doneCount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-				__threadfence_system();
-				__hip_atomic_store(doneFlag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-			}
-		}
+		if (threadIdx.x == 0)
+			__hip_atomic_store(doneFlag + blockIdx.x, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 	}
 }
 
@@ -244,18 +238,20 @@ int event_kernel_occupancy(const AmpLayout& lay, const TileShape& ts, int rank, 
 }
 
 hipError_t launch_event_reduce(const double* partials, int nWg, uint32_t nElem, double flat,
-                               double* out, hipStream_t s, unsigned int* doneCount, unsigned long long* doneFlag,
+                               double* out, hipStream_t s, unsigned long long* doneFlag,
                                unsigned long long seq, hipEvent_t start, hipEvent_t stop)
 {
-	const int nb = (int)((nElem + kRedElems - 1) / kRedElems);
+	const int nb = (int)reduce_blocks(nElem);
 	if (start || stop)
 		hipExtLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0u, s, start, stop, 0u,
-		                      partials, nWg, nElem, flat, out, doneCount, doneFlag, seq);
+		                      partials, nWg, nElem, flat, out, doneFlag, seq);
 	else
 		hipLaunchKernelGGL(event_reduce_kernel, dim3(nb), dim3(kRedElems * kRedChunks), 0, s, partials,
-		                   nWg, nElem, flat, out, doneCount, doneFlag, seq);
+		                   nWg, nElem, flat, out, doneFlag, seq);
 	return hipGetLastError();
 }
+
+uint32_t reduce_blocks(uint32_t nElem) { return (nElem + kRedElems - 1) / kRedElems; }
 
 hipError_t launch_wave_nonzero(const double* d_amps, uint64_t nEvents, uint32_t W, uint32_t wave,
                                unsigned int* flag, int nCu, hipStream_t s)

@@ -1,6 +1,7 @@
 // pwalik_capi.cc -- C ABI over rpwa::pwaLikelihood / rpwa::ampIntegralMatrix
 // (declared in include/rootpwa_amd_pwalik.h).
 #include <complex>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -535,6 +536,22 @@ extern "C" int rpwa_pwalik_fdf(const rpwa_pwalik* L, const double* par, double* 
 	if (!L || !par || !value || !gradient)
 		return fail("rpwa_pwalik_fdf", "null pointer");
 	WITH_LIK(L, lik.FdF(par, *value, gradient));
+	return RPWA_OK;
+	EVAL_GUARD_END
+}
+
+extern "C" int rpwa_pwalik_time_fdf(const rpwa_pwalik* L, const double* par, uint32_t n, double* value,
+                                    double* gradient, double* seconds)
+{
+	GUARD_BEGIN
+	if (!L || !par || !value || !gradient || !seconds || n == 0)
+		return fail("rpwa_pwalik_time_fdf", "null pointer or no call");
+	WITH_LIK(L, {
+		const auto t0 = std::chrono::steady_clock::now();
+		for (uint32_t i = 0; i < n; ++i)
+			lik.FdF(par, *value, gradient);
+		*seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	});
 	return RPWA_OK;
 	EVAL_GUARD_END
 }

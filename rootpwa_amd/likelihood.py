@@ -56,6 +56,7 @@ PROTOTYPES = {
     "rpwa_pwalik_doeval": (_i, [_vp, _dp, _dp]),
     "rpwa_pwalik_gradient": (_i, [_vp, _dp, _dp]),
     "rpwa_pwalik_fdf": (_i, [_vp, _dp, _dp, _dp]),
+    "rpwa_pwalik_time_fdf": (_i, [_vp, _dp, ctypes.c_uint32, _dp, _dp, _dp]),
     "rpwa_pwalik_doderivative": (_i, [_vp, _dp, _u32, _dp]),
     "rpwa_pwalik_hessian": (_i, [_vp, _dp, _dp]),
     "rpwa_pwalik_covariance": (_i, [_vp, _dp, _dp]),
@@ -460,6 +461,15 @@ class pwaLikelihood:
         v = ctypes.c_double()
         _call("rpwa_pwalik_fdf", self.h, _d(p), ctypes.byref(v), _d(g))
         return v.value, g
+
+    def timeFdF(self, par, n):
+        """n FdF calls at par from C++ (rpwa_pwalik_time_fdf): the native call rate
+        without this binding's per-call overhead; returns (seconds, value, gradient)."""
+        p = self._par(par)
+        g = np.zeros_like(p)
+        v, sec = ctypes.c_double(), ctypes.c_double()
+        _call("rpwa_pwalik_time_fdf", self.h, _d(p), int(n), ctypes.byref(v), _d(g), ctypes.byref(sec))
+        return sec.value, v.value, g
 
     def DoDerivative(self, par, k):
         p = self._par(par)

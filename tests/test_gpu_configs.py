@@ -73,6 +73,9 @@ def test_c0_full_size(gpu):
         assert abs(f - f_ref) <= 1e-10 * abs(f_ref)
         assert np.max(np.abs(g - g_ref)) <= 1e-10 * np.max(np.abs(g_ref))
         assert L.DoEval(par) == f
+        # the native timing loop returns the same bits (rpwa_pwalik_time_fdf)
+        sec, f2, g2 = L.timeFdF(par, 3)
+        assert f2 == f and np.array_equal(g2, g) and sec > 0
     d_data.free()
 
 
