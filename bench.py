@@ -164,6 +164,10 @@ def main():
     ap.add_argument("--initializer", action="store_true",
                     help="learned-initializer protocol only (training set of --init-train bins, best of 64 "
                          "starts; evaluation on --init-test held-out bins), printed as its own JSON line")
+    ap.add_argument("--skip-variants", action="store_true",
+                    help="skip the side-by-side timing of the complex64 staged-kernel variants")
+    ap.add_argument("--skip-initializer", action="store_true",
+                    help="skip the learned-initializer protocol (about 3.5 minutes of the default run)")
     ap.add_argument("--init-train", type=int, default=2000)
     ap.add_argument("--init-test", type=int, default=200)
     ap.add_argument("--init-epochs", type=int, default=400)
@@ -288,6 +292,7 @@ def main():
 
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
+    variants = None if args.skip_variants else variants_leg(d_data, N, nW, rank, sums, M, truth, args.steps, log)
     hess = None if args.skip_hessian else hessian_leg(L_64, L_mix, par, N, W, rank, log)
     fits = None if args.skip_fit else fit_leg(L_64, L_mix, log)
 
@@ -362,6 +367,7 @@ def main():
 
     c0 = None if args.skip_c0 else c0_leg(args, log)
     batched = None if args.skip_batch else batched_leg(args, log)
+    init_rep = None if args.skip_initializer else initializer_leg(args, log)
 
     bytes_mix = float(N) * W * 8
     bytes_64 = float(N) * W * 16
@@ -412,6 +418,8 @@ def main():
                            "note": "one DoEval + Gradient pair per eval at the same point (Minuit2 pattern)"},
         "c0": c0,
         "batched": batched,
+        "initializer": init_rep,
+        "stage_variants": variants,
         "hessian": hess,
         "fits": None if fits is None else {k: {kk: vv for kk, vv in v.items() if kk != "par"} if isinstance(v, dict)
                                            else v for k, v in fits.items()},
@@ -419,6 +427,51 @@ def main():
     if cpu:
         out["vs_cpu"] = r_mix["rate"] / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def variants_leg(d_data, N, nW, rank, sums, M, truth, steps, log):
+    """The complex64 LDS-staged kernel variants side by side on the same
+    amplitudes, in one process (kernel times differ by several per cent from
+    box to box): tiles of 64 and 32 events, gradient phase on FP64 VALU or on
+    FP32 MFMA; event-kernel time (HIP events bound to the launches) and FdF
+    rate of the C ABI, lnL and gradient against the FP64-VALU variant."""
+    from rootpwa_amd import capi
+    norm = np.sqrt(np.real(np.diag(sums)) / M)
+    prod = np.asarray(truth) * 1.1
+    ctxs = {}
+    for events, gm in ((64, False), (64, True), (32, False), (32, True)):
+        c = capi.LikelihoodContext(0, capi.PREC_MIXED)
+        c.load_device(d_data.address, N, nW, norm)
+        c.set_stage_variant(events, gm)
+        c.set_timing(True)
+        ctxs["%d_events_%s" % (events, "mfma" if gm else "valu")] = c
+    ref = None
+    out = {}
+    for rep in range(3):
+        for name, c in ctxs.items():
+            for _ in range(3):
+                f, d, df = c.fdf(prod, 0.1)
+            if ref is None:
+                ref = (f, d)
+            c.reset_kernel_time()
+            capi.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                c.fdf(prod, 0.1)
+            el = (time.perf_counter() - t) / steps
+            ev, red, n = c.kernel_times()
+            o = out.setdefault(name, {"kernel_ms": [], "evals_per_s": []})
+            o["kernel_ms"].append(1e3 * ev / n)
+            o["evals_per_s"].append(1.0 / el)
+            o["lnL_minus_first"] = f - ref[0]
+            o["grad_vs_first"] = float(np.max(np.abs(d - ref[1])) / np.max(np.abs(ref[1])))
+    for name, o in out.items():
+        o["kernel_ms"] = float(np.median(o["kernel_ms"]))
+        o["evals_per_s"] = float(np.median(o["evals_per_s"]))
+        log("staged variant %s: kernel %.4f ms, %.1f evals/s" % (name, o["kernel_ms"], o["evals_per_s"]))
+    for c in ctxs.values():
+        c.close()
+    return out
 
 
 def fit_leg(L_64, L_mix, log):

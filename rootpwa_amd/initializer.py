@@ -164,17 +164,19 @@ def starts_needed(values, best, tol=0.5):
 
 
 def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5, chunk=48, log=None, device=0):
-    """Held-out comparison: per bin, nRandom random starts vs the network's
-    prediction plus the first nMixed - 1 of those random starts.  Reports the
-    starts and evaluations needed to come within 0.5 of the best lnL found by
-    either, and the wall time of the two batched fits."""
+    """Held-out comparison, three arms per bin: nRandom random starts; the
+    network's prediction plus the first nMixed - 1 of those random starts; the
+    spectral estimate alone (the network's input, without the network) plus the
+    same nMixed - 1 random starts.  Reports the starts and evaluations needed to
+    come within 0.5 of the best lnL found by any arm, and the wall time of the
+    batched fits."""
     rows = []
-    tR = tM = 0.0
+    tR = tM = tS = 0.0
     for c0 in range(0, len(sizes), chunk):
         part = sizes[c0:c0 + chunk]
         bfR = B.BatchFit(W, nRandom, device)
         bfM = B.BatchFit(W, nMixed, device)
-        bfS = B.BatchFit(W, 1, device)
+        bfS = B.BatchFit(W, nMixed, device)
         moments, nev = [], []
         for k, n in enumerate(part):
             d_data, sums, M, _ = synth.generate_bin(synth.bin_seed(seed, c0 + k), W, int(n), mcFactor, device)
@@ -199,14 +201,21 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
         t = time.perf_counter()
         parM, valM, stM, neM = bfM.fit(startPar=mixed)
         tM += time.perf_counter() - t
-        parS, valS, stS, neS = bfS.fit(startPar=init.predict(moments, nev, spectralOnly=True)[:, None, :])
+        spectral = np.empty_like(mixed)
+        spectral[:, 0] = init.predict(moments, nev, spectralOnly=True)
+        spectral[:, 1:] = rnd[:, :nMixed - 1]
+        t = time.perf_counter()
+        parS, valS, stS, neS = bfS.fit(startPar=spectral)
+        tS += time.perf_counter() - t
         for k in range(len(part)):
             best = min(valR[k].min(), valM[k].min(), valS[k].min())
             sR = starts_needed(valR[k], best)
             sM = starts_needed(valM[k], best)
-            rows.append(dict(events=nev[k], best=best, startsRandom=sR, startsMixed=sM,
+            sS = starts_needed(valS[k], best)
+            rows.append(dict(events=nev[k], best=best, startsRandom=sR, startsMixed=sM, startsSpectral=sS,
                              evalsRandom=int(neR[k][:min(sR, nRandom)].sum()),
                              evalsMixed=int(neM[k][:min(sM, nMixed)].sum()),
+                             evalsSpectral=int(neS[k][:min(sS, nMixed)].sum()),
                              mlpValue=float(valM[k][0]), mlpEvals=int(neM[k][0]),
                              mlpStartGap=float(startVal[k][0] - best),
                              randomStartGap=float(np.median(startVal[k][1:] - best)),
@@ -217,24 +226,30 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
         del bfR, bfM, bfS
     sR = np.array([r["startsRandom"] for r in rows])
     sM = np.array([r["startsMixed"] for r in rows])
+    sS = np.array([r["startsSpectral"] for r in rows])
     eR = np.array([r["evalsRandom"] for r in rows])
     eM = np.array([r["evalsMixed"] for r in rows])
+    eS = np.array([r["evalsSpectral"] for r in rows])
     return {
         "bins": len(rows),
         "starts_needed": {"random": float(sR.mean()), "mlp_plus_random": float(sM.mean()),
-                          "ratio": float(sR.mean() / sM.mean())},
+                          "spectral_plus_random": float(sS.mean()), "ratio": float(sR.mean() / sM.mean())},
         "reached_within_budget": {"random_%d" % nRandom: float(np.mean(sR <= nRandom)),
-                                  "mlp_plus_%d" % (nMixed - 1): float(np.mean(sM <= nMixed))},
+                                  "mlp_plus_%d" % (nMixed - 1): float(np.mean(sM <= nMixed)),
+                                  "spectral_plus_%d" % (nMixed - 1): float(np.mean(sS <= nMixed))},
         "mlp_start_reaches_best": float(np.mean(sM == 1)),
         "first_random_start_reaches_best": float(np.mean(sR == 1)),
         "start_gap_median": {"mlp": float(np.median([r["mlpStartGap"] for r in rows])),
                              "random": float(np.median([r["randomStartGap"] for r in rows]))},
         "evaluations_needed": {"random": float(eR.mean()), "mlp_plus_random": float(eM.mean()),
-                               "ratio": float(eR.mean() / max(eM.mean(), 1e-9))},
+                               "spectral_plus_random": float(eS.mean()),
+                               "ratio": float(eR.mean() / max(eM.mean(), 1e-9)),
+                               "mlp_over_spectral": float(eM.mean() / max(eS.mean(), 1e-9))},
         "evaluations_per_fit": {"random_start": float(np.mean([r["randomEvalsMean"] for r in rows])),
                                 "mlp_start": float(np.mean([r["mlpEvals"] for r in rows])),
                                 "spectral_start_without_network": float(np.mean([r["spectralEvals"] for r in rows]))},
         "spectral_start_without_network_reaches_best": float(np.mean([r["spectralReaches"] for r in rows])),
-        "wall_s": {"random_%d_starts" % nRandom: tR, "mlp_plus_%d" % (nMixed - 1): tM, "ratio": tR / max(tM, 1e-9)},
+        "wall_s": {"random_%d_starts" % nRandom: tR, "mlp_plus_%d" % (nMixed - 1): tM,
+                   "spectral_plus_%d" % (nMixed - 1): tS, "ratio": tR / max(tM, 1e-9)},
         "per_bin": rows,
     }

@@ -6,7 +6,7 @@ data and integral.  The batch stores amplitudes as complex64 and contracts on
 FP32 MFMA with the production amplitudes rounded to FP32; the test points are
 therefore FP32-representable, so the oracle sees exactly the production
 amplitudes the kernel uses and the differences measure only the complex64
-amplitude storage and the FP32 contraction.  Tolerances: |d lnL| <= 1e-6 (sqrt(N W / 8) + 1)
+amplitude storage and the FP32 contraction.  Tolerances: |d lnL| <= min(1e-6 (sqrt(N W / 8) + 1), 1e-3)
 (absolute, per fit, N data events, W waves: the FP32 rounding of each event's
 sigma grows as the root of the 2 W summed terms and the events' errors add as
 a random walk) and gradient max-abs error <= 1e-5 of the gradient's
@@ -46,7 +46,8 @@ def fp32_params(rng, nb, S, P, scale):
 
 
 def f_tol(n, W):
-    return 1e-6 * (np.sqrt(n * W / 8.0) + 1.0)
+    # never looser than the 1e-3 the single-fit complex64 path is held to
+    return min(1e-6 * (np.sqrt(n * W / 8.0) + 1.0), 1e-3)
 
 
 def check_against_oracle(bf, bins, par, starts=None):
