@@ -433,18 +433,20 @@ def variants_leg(d_data, N, nW, rank, sums, M, truth, steps, log):
     """The complex64 LDS-staged kernel variants side by side on the same
     amplitudes, in one process (kernel times differ by several per cent from
     box to box): tiles of 64 and 32 events, gradient phase on FP64 VALU or on
-    FP32 MFMA; event-kernel time (HIP events bound to the launches) and FdF
-    rate of the C ABI, lnL and gradient against the FP64-VALU variant."""
+    FP32 MFMA, and the wave-tile variant (S on FP64 MFMA, gradient on FP32
+    MFMA, one wave per 16 events); event-kernel time (HIP events bound to the
+    launches) and FdF rate of the C ABI, lnL and gradient against the first."""
     from rootpwa_amd import capi
     norm = np.sqrt(np.real(np.diag(sums)) / M)
     prod = np.asarray(truth) * 1.1
     ctxs = {}
-    for events, gm in ((64, False), (64, True), (32, False), (32, True)):
+    names = {0: "valu", 1: "mfma_gradient", 2: "wave_tile_mfma"}
+    for events, variant in ((64, 0), (64, 1), (64, 2), (32, 0), (32, 1)):
         c = capi.LikelihoodContext(0, capi.PREC_MIXED)
         c.load_device(d_data.address, N, nW, norm)
-        c.set_stage_variant(events, gm)
+        c.set_stage_variant(events, variant)
         c.set_timing(True)
-        ctxs["%d_events_%s" % (events, "mfma" if gm else "valu")] = c
+        ctxs["%d_events_%s" % (events, names[variant])] = c
     ref = None
     out = {}
     for rep in range(3):

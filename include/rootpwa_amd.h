@@ -170,14 +170,18 @@ int rpwa_lik_set_tile(rpwa_lik_ctx* ctx, uint32_t lanesPerEvent);
  * No counterpart in the reference: a tuning and testing control like
  * rpwa_lik_set_tile, for the tile sizes the autotuner chooses between. */
 int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile);
-/* As rpwa_lik_set_stage_tile, choosing also the gradient phase: on FP32 MFMA
- * (gradientOnMfma != 0; complex64 amplitudes, rows of at most 128 waves, 64 or
- * 32 events per tile) or FP64 VALU (0).  No counterpart in the reference. */
-int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gradientOnMfma);
+/* As rpwa_lik_set_stage_tile, choosing also the variant: 0 = S and the
+ * gradient on FP64 VALU; 1 = the gradient phase on FP32 MFMA (complex64
+ * amplitudes, rows of at most 128 waves, 64 or 32 events per tile); 2 = the
+ * wave-tile variant, each wave computing S of 16 events on FP64 MFMA, sigma and
+ * u inside the wave and their gradient on FP32 MFMA (complex64, rows of at most
+ * 96 waves, 64 events per tile; ranks 1 and 2, higher ranks keep variant 0).
+ * No counterpart in the reference. */
+int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int variant);
 /* The event-kernel variant of a rank: variant[0] lanes per event (1: LDS-staged),
- * [1] slots per lane (events per tile for the staged kernel), [2] 1 if the
- * staged kernel's gradient phase runs on MFMA, [3] workgroups.  No counterpart
- * in the reference. */
+ * [1] slots per lane (events per tile for the staged kernel), [2] the staged
+ * variant (0, 1 or 2 as in rpwa_lik_set_stage_variant), [3] workgroups.  No
+ * counterpart in the reference. */
 int rpwa_lik_kernel_variant(rpwa_lik_ctx* ctx, uint32_t rank, uint32_t variant[4]);
 
 /* Stream on which the context launches its kernels (for event timing). */

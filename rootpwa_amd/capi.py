@@ -252,8 +252,10 @@ class LikelihoodContext:
     def set_stage_tile(self, events):
         call("rpwa_lik_set_stage_tile", self.h, int(events))
 
-    def set_stage_variant(self, events, gradient_on_mfma):
-        call("rpwa_lik_set_stage_variant", self.h, int(events), int(bool(gradient_on_mfma)))
+    def set_stage_variant(self, events, variant):
+        """variant 0: VALU S and gradient; 1 (or True): gradient on FP32 MFMA;
+        2: wave-tile variant (S on FP64 MFMA, gradient on FP32 MFMA)."""
+        call("rpwa_lik_set_stage_variant", self.h, int(events), int(variant))
 
     def kernel_variant(self, rank):
         out = (ctypes.c_uint32 * 4)()

@@ -316,7 +316,9 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 	// more tiles per workgroup)
 	// (and, for complex64 rows of at most 64 units, the staged kernel with the
 	// gradient phase on FP32 MFMA, tiles of 64 and 32 events)
-	const int cands[][2] = {{8, 0}, {16, 0}, {32, 0}, {64, 0}, {-64, 0}, {-32, 0}, {-16, 0}, {-64, 1}, {-32, 1}};
+	// (and the wave-tile variant: S and the gradient on MFMA, one wave per 16 events, 64-event tiles)
+	const int cands[][2] = {{8, 0}, {16, 0}, {32, 0}, {64, 0}, {-64, 0}, {-32, 0}, {-16, 0}, {-64, 1}, {-32, 1},
+	                        {-64, 2}};
 	for (const auto& cg : cands) {
 			const int cand = cg[0], gm = cg[1];
 			TileShape ts;
@@ -347,7 +349,9 @@ int autotune_rank(rpwa_lik_ctx* c, int rank)
 			if (std::getenv("RPWA_VERBOSE"))
 				std::fprintf(stderr, "[rpwa autotune] precision %d rank %d lanes %d slots %d (%d+%d lanes) "
 				             "events/tile %d%s workgroups %d: %.4f ms\n", c->precision, rank, ts.lanes, ts.slots,
-				             ts.lanes0, ts.lanes1, ts.events, ts.gm ? " (gradient on MFMA)" : "", c->nWg[rank], med);
+				             ts.lanes0, ts.lanes1, ts.events,
+				             ts.gm == 2 ? " (S and gradient on MFMA)" : ts.gm ? " (gradient on MFMA)" : "",
+				             c->nWg[rank], med);
 			if (med < bestT) {
 				bestT = med;
 				best = ts;
@@ -895,18 +899,23 @@ int set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gm, const c
 		return RPWA_ERR_INVALID;
 	}
 	TileShape ts;
-	for (int r = 1; r <= kMaxRank; ++r)
+	// the wave-tile variant covers ranks 1 and 2; higher ranks keep the VALU staged kernel
+	const int rTop = gm >= 2 ? 2 : kMaxRank;
+	for (int r = 1; r <= rTop; ++r)
 		if (!make_stage_tile(ctx->lay, r, ts, e, gm) || ts.events != e || ts.gm != gm) {
 			set_error(std::string(fn) + ": a tile of " + std::to_string(e) + " amplitude rows does not fit the " +
-			          (gm ? "LDS-staged kernel with the gradient on MFMA (complex64 rows of at most 128 waves, "
-			                "tiles of 64 or 32 events)"
-			              : "LDS-staged kernel"));
+			          (gm == 2 ? "wave-tile variant (complex64 rows of at most 96 waves, ranks up to 2, "
+			                     "tiles of 64 events)"
+			           : gm ? "LDS-staged kernel with the gradient on MFMA (complex64 rows of at most 128 waves, "
+			                  "tiles of 64 or 32 events)"
+			                : "LDS-staged kernel"));
 			return RPWA_ERR_UNSUPPORTED;
 		}
 	RPWA_GUARD_BEGIN
 	RPWA_HIP_CHECK(hipSetDevice(ctx->device));
 	for (int r = 1; r <= kHighRankSlot; ++r) {
-		make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts, e, gm);
+		if (!make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts, e, r <= rTop ? gm : 0))
+			make_stage_tile(ctx->lay, std::min(r, kMaxRank), ts);
 		ctx->tile[r] = ts;
 		ctx->tileFixed[r] = true;
 		if (int st = configure_grid(ctx, r))
@@ -922,9 +931,14 @@ extern "C" int rpwa_lik_set_stage_tile(rpwa_lik_ctx* ctx, uint32_t eventsPerTile
 	return set_stage_variant(ctx, eventsPerTile, 0, "rpwa_lik_set_stage_tile");
 }
 
-extern "C" int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int gradientOnMfma)
+extern "C" int rpwa_lik_set_stage_variant(rpwa_lik_ctx* ctx, uint32_t eventsPerTile, int variant)
 {
-	return set_stage_variant(ctx, eventsPerTile, gradientOnMfma ? 1 : 0, "rpwa_lik_set_stage_variant");
+	if (variant < 0 || variant > 2) {
+		clear_error();
+		set_error("rpwa_lik_set_stage_variant: variant must be 0, 1 or 2");
+		return RPWA_ERR_INVALID;
+	}
+	return set_stage_variant(ctx, eventsPerTile, variant, "rpwa_lik_set_stage_variant");
 }
 
 extern "C" int rpwa_lik_kernel_variant(rpwa_lik_ctx* ctx, uint32_t rank, uint32_t variant[4])

@@ -365,12 +365,13 @@ __device__ __forceinline__ void g_phase(const StageArgs& a, const f32x4* stage, 
 // in both partners, and the rounding error of each addition (TwoSum) goes to
 // the compensation ----
 template <int E>
-__device__ __forceinline__ void write_log_sums(const StageArgs& a, double* out, int lane, double mant, int expo,
-                                               double wsum)
+__device__ __forceinline__ void log_sums_butterfly(int lane, double mant, int expo, double wsum, double& s, double& c,
+                                                   double& ws)
 {
 	const bool ev = E == 64 || lane < E;
-	double s = ev ? -(log(mant) + (double)expo * 0.69314718055994530942) : 0.;
-	double c = 0., ws = ev ? wsum : 0.;
+	s = ev ? -(log(mant) + (double)expo * 0.69314718055994530942) : 0.;
+	c = 0.;
+	ws = ev ? wsum : 0.;
 #pragma unroll
 	for (int m = 32; m >= 1; m >>= 1) {
 		const double so = __shfl_xor(s, m), co = __shfl_xor(c, m), wo = __shfl_xor(ws, m);
@@ -383,6 +384,14 @@ __device__ __forceinline__ void write_log_sums(const StageArgs& a, double* out, 
 		ws = upper ? wo + ws : ws + wo;
 		s = t;
 	}
+}
+
+template <int E>
+__device__ __forceinline__ void write_log_sums(const StageArgs& a, double* out, int lane, double mant, int expo,
+                                               double wsum)
+{
+	double s, c, ws;
+	log_sums_butterfly<E>(lane, mant, expo, wsum, s, c, ws);
 	if (lane == 0 && a.rank0 == 0) {
 		out[0] = s;
 		out[1] = c;
@@ -390,13 +399,44 @@ __device__ __forceinline__ void write_log_sums(const StageArgs& a, double* out, 
 	}
 }
 
+// every wave holds log products of its own events: the waves' butterfly sums
+// are added in wave order (TwoSum into the compensation) by thread 0; lsum is
+// a [waves][3] LDS area; ends with a workgroup barrier
+__device__ __forceinline__ void write_log_sums_all_waves(const StageArgs& a, double* out, double (*lsum)[3], int tid,
+                                                         int lane, int wid, double mant, int expo, double wsum)
+{
+	double s, c, ws;
+	log_sums_butterfly<64>(lane, mant, expo, wsum, s, c, ws);
+	if (lane == 0) {
+		lsum[wid][0] = s;
+		lsum[wid][1] = c;
+		lsum[wid][2] = ws;
+	}
+	lds_barrier();
+	if (tid == 0 && a.rank0 == 0) {
+		double S = lsum[0][0], C = lsum[0][1], W = lsum[0][2];
+#pragma unroll
+		for (int w = 1; w < kWaves; ++w) {
+			const double x = S, y = lsum[w][0];
+			const double t = x + y;
+			const double yp = t - x;
+			C = (C + lsum[w][1]) + ((x - (t - yp)) + (y - yp));
+			W += lsum[w][2];
+			S = t;
+		}
+		out[0] = S;
+		out[1] = C;
+		out[2] = W;
+	}
+}
+
 // ---- end of the MFMA variant: every wave's FP64 sums D[unit][column][row] go
 // to LDS, then each gradient entry is formed from them, waves in order:
 // u conj(x) = (u_re x_re + u_im x_im) + i (u_im x_re - u_re x_im) ----
-template <int R, int E, int NG, int NT>
+template <int R, int E, int NG, int NT, bool ALLW = false>
 __device__ __forceinline__ void write_partial_row_gm(const StageArgs& a, f32x4* smem, int tid, int lane, int wid,
                                                      const double (&GD)[NG][gm_col_sets<R>()][4], double mant,
-                                                     int expo, double wsum)
+                                                     int expo, double wsum, double (*lsum)[3] = nullptr)
 {
 	constexpr int NCS = gm_col_sets<R>();
 	constexpr int NCOL = 4 * NCS;
@@ -413,9 +453,13 @@ __device__ __forceinline__ void write_partial_row_gm(const StageArgs& a, f32x4* 
 				p[i] = GD[g][c][i];
 		}
 	double* out = a.partials + (size_t)blockIdx.x * a.nElem;
-	if (wid == 0)
-		write_log_sums<E>(a, out, lane, mant, expo, wsum);
-	lds_barrier();
+	if constexpr (ALLW) {
+		write_log_sums_all_waves(a, out, lsum, tid, lane, wid, mant, expo, wsum);
+	} else {
+		if (wid == 0)
+			write_log_sums<E>(a, out, lane, mant, expo, wsum);
+		lds_barrier();
+	}
 	const uint32_t nCplx = (a.nElem - 3) / 2;
 	for (uint32_t idx = tid; idx < nCplx; idx += NT) {
 		const uint32_t j = idx % a.wmax;
@@ -648,6 +692,198 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 		write_partial_row<MIXED, R, E, kThreads>(a, smem, tid, lane, wid, gOn, gU0, gEnd, gSub, G, mant, expo, wsum);
 }
 
+// ---- The wave-tile variant (complex64, rank <= 2, rows of at most 16 NG units,
+// tiles of 64 events): each wave takes 16 events of the staged tile and computes
+// their complete S on v_mfma_f64_4x4x4f64, sigma and u inside the wave, and the
+// gradient of those events on v_mfma_f32_4x4x1f32 -- two barriers per tile (the
+// stage) and no single-wave step.
+//
+// S: 4 blocks of 4x4x4 per MFMA, K = the four floats of one 16-byte unit (Re, Im
+// of its two waves), block b and row i = event 4b + i of the wave's 16, column
+// j = (Re S, Im S) of rank 0, then rank 1.  Layout on gfx950: A[b][i][k] in lane
+// 16k + 4b + i, B[b][k][j] in lane 16k + 4b + j, D[b][i][j] in lane 16i + 4b + j.
+// B is T re-laid as a real 4x4 matrix per unit, the same for every block, kept
+// in LDS after the stage for the whole kernel (16 doubles per unit):
+//   Re S += Re x Re T - Im x Im T,  Im S += Re x Im T + Im x Re T.
+// The two sectors have their own accumulators; sigma of an event is the sum of
+// its four lanes' squares (a quad, combined by symmetric exchanges so that all
+// four hold the same bits).
+// S of 16 events over units [u0, u1) on v_mfma_f64_4x4x4f64 in four independent
+// accumulator chains (units u0 + 4m + c go to chain c), added in a fixed order
+// at the end: the chains keep the matrix pipe busy where one chain would wait
+// for each result
+__device__ __forceinline__ double s_mfma(const float* xa, const double* tLane, uint32_t u0, uint32_t u1)
+{
+	double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+	uint32_t u = u0;
+	for (; u + 4 <= u1; u += 4) {
+		const double x0 = (double)xa[u * 4], x1 = (double)xa[u * 4 + 4];
+		const double x2 = (double)xa[u * 4 + 8], x3 = (double)xa[u * 4 + 12];
+		const double t0 = tLane[u * 16], t1 = tLane[u * 16 + 16], t2 = tLane[u * 16 + 32], t3 = tLane[u * 16 + 48];
+		c0 = __builtin_amdgcn_mfma_f64_4x4x4f64(x0, t0, c0, 0, 0, 0);
+		c1 = __builtin_amdgcn_mfma_f64_4x4x4f64(x1, t1, c1, 0, 0, 0);
+		c2 = __builtin_amdgcn_mfma_f64_4x4x4f64(x2, t2, c2, 0, 0, 0);
+		c3 = __builtin_amdgcn_mfma_f64_4x4x4f64(x3, t3, c3, 0, 0, 0);
+	}
+	if (u < u1)
+		c0 = __builtin_amdgcn_mfma_f64_4x4x4f64((double)xa[u * 4], tLane[u * 16], c0, 0, 0, 0);
+	if (u + 1 < u1)
+		c1 = __builtin_amdgcn_mfma_f64_4x4x4f64((double)xa[u * 4 + 4], tLane[u * 16 + 16], c1, 0, 0, 0);
+	if (u + 2 < u1)
+		c2 = __builtin_amdgcn_mfma_f64_4x4x4f64((double)xa[u * 4 + 8], tLane[u * 16 + 32], c2, 0, 0, 0);
+	return (c0 + c1) + (c2 + c3);
+}
+
+// G += u conj(x) of 16 staged events (rows at rows + e * rowBytes) on
+// v_mfma_f32_4x4x1f32, one accumulator chain per unit group, added into the
+// FP64 sums GD at the end of the block
+template <int NG>
+__device__ __forceinline__ void g_mfma_block(const char* rows, uint32_t rowBytes, const float* uw,
+                                             const uint32_t (&aOff)[NG], const uint32_t (&bOff)[NG],
+                                             double (&GD)[NG][1][4])
+{
+	constexpr int EW = 16;
+	gm_f4 acc[NG];
+#pragma unroll
+	for (int g = 0; g < NG; ++g)
+		acc[g] = gm_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+	for (int q4 = 0; q4 < EW; q4 += 4) {
+		gm_f4 bv[NG];
+#pragma unroll
+		for (int g = 0; g < NG; ++g)
+			bv[g] = *reinterpret_cast<const gm_f4*>(uw + bOff[g] + q4);
+#pragma unroll
+		for (int kk = 0; kk < 4; ++kk) {
+			const char* row = rows + (uint32_t)(q4 + kk) * rowBytes;
+#pragma unroll
+			for (int g = 0; g < NG; ++g) {
+				const float av = *reinterpret_cast<const float*>(row + aOff[g]);
+				acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv[g][kk], acc[g], 0, 0, 0);
+			}
+		}
+	}
+#pragma unroll
+	for (int g = 0; g < NG; ++g)
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+			GD[g][0][i] += (double)acc[g][i];
+}
+
+template <int R, int NG>
+__global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
+{
+	constexpr int E = 64;
+	constexpr int NU = kGmUnits * NG;   // units of T' in LDS (rows of at most NU units)
+	constexpr int EW = 16;              // events per wave
+	extern __shared__ f32x4 smem[];
+	__shared__ double lsum[kWaves][3];
+	f32x4* stage = smem;  // [E][rowUnitsP] units, then the dump unit
+
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+	// u of every wave's events: [wave][sector][column][16] floats after the stage,
+	// then T' [unit][k][j] doubles (B of the S MFMA)
+	float* uw = reinterpret_cast<float*>(reinterpret_cast<char*>(smem) + a.stageBytes) + wid * (2 * 4 * EW);
+	double* tL = reinterpret_cast<double*>(reinterpret_cast<char*>(smem) + a.stageBytes + kWaves * 2 * 4 * EW * 4);
+	uint32_t ldsPos[kMaxLoads], ldOff[kMaxLoads];
+	load_plan<kThreads, kMaxLoads, E>(a, tid, ldsPos, ldOff);
+	{
+		typedef const __attribute__((address_space(4))) char cchar;
+		cdouble* tp = a.tInline ? (cdouble*)((cchar*)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(StageArgs, tin))
+		                        : (cdouble*)(const double*)a.tpad;
+		for (uint32_t idx = (uint32_t)tid; idx < (uint32_t)NU * 16u; idx += kThreads) {
+			const uint32_t u = idx / 16u, k = (idx / 4u) % 4u, j = idx % 4u;
+			const uint32_t r = j >> 1;
+			double v = 0.;
+			if (u < a.rowUnits && r < (uint32_t)R) {
+				const uint32_t col = 2u * u + (k >> 1);
+				const double tre = tp[2 * (col * R + r)], tim = tp[2 * (col * R + r) + 1];
+				if ((j & 1) == 0)
+					v = (k & 1) == 0 ? tre : -tim;  // Re S
+				else
+					v = (k & 1) == 0 ? tim : tre;   // Im S
+			}
+			tL[idx] = v;
+		}
+	}
+	// this lane's entry of T' in every unit: T'[k = lane / 16][j = lane % 4]
+	const double* tLane = tL + (lane >> 4) * 4 + (lane & 3);
+	// G MFMA: per unit group the lane's byte offset in a staged row and the
+	// float offset of its B operand in uw (sector of the unit, column lane % 4)
+	uint32_t aOff[NG], bOff[NG];
+	double GD[NG][1][4];
+#pragma unroll
+	for (int g = 0; g < NG; ++g) {
+		const uint32_t unit = (uint32_t)(g * kGmUnits + lane / 4);
+		aOff[g] = min(unit, a.rowUnits - 1) * 16u + (uint32_t)(lane % 4) * 4u;
+		const uint32_t sec = unit >= a.secUnit ? 1u : 0u;
+		bOff[g] = (sec * 4u + (uint32_t)(lane % 4)) * (uint32_t)EW;
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+			GD[g][0][i] = 0.;
+	}
+	double mant = 1., wsum = 0.;
+	int expo = 0;
+	const uint32_t rowBytes = a.rowUnitsP * 16u;
+	// S operand A: event (lane % 16) of the wave's 16, float (lane / 16) of each unit
+	const uint32_t aRow = ((uint32_t)(wid * EW + (lane & 15))) * rowBytes + (uint32_t)(lane >> 4) * 4u;
+	// D layout: this lane's event of the wave's 16 and its column
+	const int dEv = 4 * ((lane >> 2) & 3) + (lane >> 4);
+	const int dCol = lane & 3;
+
+	f32x4 buf[kMaxLoads];
+	const uint32_t step = gridDim.x;
+	const uint32_t lastTile = a.nTiles ? a.nTiles - 1 : 0;
+	uint32_t tile = blockIdx.x;
+	if (tile < a.nTiles) {
+		fetch_tile(a, tile, lastTile, ldOff, buf);
+		const uint32_t nMine = (a.nTiles - 1 - tile) / step + 1;
+		for (uint32_t it = 0; it < nMine; ++it, tile += step) {
+			lds_barrier();  // every wave is done with the previous tile
+			store_tile(stage, ldsPos, buf);
+			fetch_tile(a, min(tile + step, lastTile), lastTile, ldOff, buf);
+			lds_barrier();
+			const char* sb = reinterpret_cast<const char*>(stage);
+			// ---- S of the wave's 16 events ----
+			const float* xa = reinterpret_cast<const float*>(sb + aRow);
+			const double s0 = s_mfma(xa, tLane, 0, a.secUnit);
+			const double s1 = s_mfma(xa, tLane, a.secUnit, a.rowUnits);
+			// ---- sigma, the log product, the weights and u, inside the wave ----
+			double q = s0 * s0 + s1 * s1;
+			q = q + __shfl_xor(q, 1);
+			q = q + __shfl_xor(q, 2);
+			const double sigma = q + a.flat2;
+			const uint64_t ev = (uint64_t)tile * E + (uint64_t)(wid * EW + dEv);
+			double w = 0.;
+			if (ev < a.nEvents) {
+				double rc = __builtin_amdgcn_rcp(sigma);
+				rc = fma(fma(-sigma, rc, 1.), rc, rc);
+				w = -2. * rc;
+				if (dCol == 0) {
+					mant *= sigma;
+					const uint64_t ub = (uint64_t)__double_as_longlong(mant);
+					const int ex = (int)((ub >> 52) & 0x7FF);
+					if (ex != 0 && ex != 0x7FF) {
+						expo += ex - 1023;
+						mant = __longlong_as_double((long long)((ub & 0x800FFFFFFFFFFFFFull) | (1023ull << 52)));
+					}
+					wsum += w;
+				}
+			}
+			uw[(0 * 4 + dCol) * EW + dEv] = (float)(w * s0);
+			uw[(1 * 4 + dCol) * EW + dEv] = (float)(w * s1);
+			// the wave's own u is read back below: LDS operations of a wave complete in order
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			// ---- G of the wave's 16 events on the matrix cores ----
+			g_mfma_block<NG>(sb + (uint32_t)(wid * EW) * rowBytes, rowBytes, uw, aOff, bOff, GD);
+		}
+	}
+	write_partial_row_gm<R, E, NG, kThreads, true>(a, smem, tid, lane, wid, GD, mant, expo, wsum, lsum);
+}
+
 using StageKernel = void (*)(StageArgs);
 
 template <bool MIXED, int E, int NG>
@@ -681,8 +917,22 @@ StageKernel pick_gm(int groups, int rank)
 	}
 }
 
+template <int R>
+StageKernel pick_wt(int groups)
+{
+	switch (groups) {
+	case 1: return wave_tile_kernel<R, 1>;
+	case 2: return wave_tile_kernel<R, 2>;
+	default: return wave_tile_kernel<R, 3>;
+	}
+}
+
 StageKernel pick_stage(const AmpLayout& lay, int events, int gm, int rank)
 {
+	if (gm == 2) {
+		const int groups = (int)((lay.rowLen * 8u / 16u + kGmUnits - 1) / kGmUnits);
+		return rank <= 1 ? pick_wt<1>(groups) : pick_wt<2>(groups);
+	}
 	if (gm) {
 		const int groups = (int)((lay.rowLen * 8u / 16u + kGmUnits - 1) / kGmUnits);
 		return events == 64 ? pick_gm<64>(groups, rank) : pick_gm<32>(groups, rank);
@@ -703,8 +953,13 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p, int gm
 {
 	if (events != 16 && events != 32 && events != 64)
 		return false;
-	// the MFMA gradient: complex64 rows of at most 64 units, tiles of 64 or 32 events
+	// the MFMA gradient: complex64 rows of at most 64 units, tiles of 64 or 32 events;
+	// the wave-tile variant (S on MFMA too): rows of at most 48 units, rank <= 2, 64 events
 	if (gm && (lay.precision != RPWA_PREC_MIXED || events == 16 || lay.rowLen * 8u > 16u * kGmUnits * kGmMaxGroups))
+		return false;
+	if (gm == 2 && (events != 64 || rank > 2 || lay.rowLen * 8u > 16u * kGmUnits * 3u))
+		return false;
+	if (gm < 0 || gm > 2)
 		return false;
 	p.gm = gm;
 	const int threads = kThreads;
@@ -747,6 +1002,12 @@ bool stage_plan(const AmpLayout& lay, int events, int rank, StagePlan& p, int gm
 		const size_t gmBytes = (size_t)waves * groups * kGmUnits * 4 * ncs * 4 * sizeof(double);
 		p.ldsBytes = std::max(p.ldsBytes, gmBytes);
 	}
+	if (gm == 2) {  // the stage, every wave's u [2][4][16] floats, T' [units][4][4] doubles
+		const uint32_t units = (p.rowUnits + kGmUnits - 1) / kGmUnits * kGmUnits;
+		p.ldsBytes = std::max(p.ldsBytes, (size_t)p.stageBytes + (size_t)waves * 2 * 4 * 16 * sizeof(float) +
+		                                      (size_t)units * 16 * sizeof(double));
+	}
+
 	return p.ldsBytes <= 160u * 1024u;
 }
 

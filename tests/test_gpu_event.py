@@ -93,26 +93,30 @@ def test_every_tile_shape_agrees(gpu, case, prec):
         assert ctx.log_likelihood(p2, 0.1) == f
 
 
-@pytest.mark.parametrize("events", [64, 32])
-def test_stage_gradient_on_mfma(gpu, case, events):
-    """The staged kernel with its gradient phase on FP32 MFMA (complex64 rows of
-    at most 128 waves) against the oracle and against the FP64-VALU gradient
-    of the same tile: the value is computed by the same FP64 code (bitwise
-    equal when the grids agree), the gradient within 1e-6 of its largest entry;
-    sign flip of a (rank, sector) block and the value-only call bitwise."""
+@pytest.mark.parametrize("events,variant", [(64, 1), (32, 1), (64, 2)], ids=["g64", "g32", "wave64"])
+def test_stage_gradient_on_mfma(gpu, case, events, variant):
+    """The staged kernel with its gradient phase on FP32 MFMA (variant 1:
+    complex64 rows of at most 128 waves) or with S on FP64 MFMA and the
+    gradient on FP32 MFMA, one wave per 16 events (variant 2: rows of at most
+    96 waves, ranks 1-2; rank 3 falls back to variant 0) against the oracle and
+    against the FP64-VALU variant of the same tile: the value within the
+    complex64 bound (bitwise equal for variant 1 when the grids agree), the
+    gradient within 1e-6 of its largest entry; sign flip of a (rank, sector)
+    block and the value-only call bitwise."""
     ds, L, amps = case
     nW, rank = ds["nW"], ds["rank"]
     ctx = gpu.LikelihoodContext(0, 1)
     ctx.load(amps, nW)
     prod = H.random_prod(np.random.default_rng(31), rank, nW, 0.9)
+    wpad = sum(nW) + (nW[0] % 2) + (nW[1] % 2)
     try:
-        ctx.set_stage_variant(events, True)
+        ctx.set_stage_variant(events, variant)
     except gpu.RpwaError as e:
-        assert e.code == gpu.RPWA_ERR_UNSUPPORTED and sum(nW) + (nW[0] % 2) + (nW[1] % 2) > 128
+        assert e.code == gpu.RPWA_ERR_UNSUPPORTED and wpad > {1: 128, 2: 96}[variant]
         ctx.close()
         return
-    print("mfma gradient", nW, "rank", rank, "tile", events, flush=True)
-    assert ctx.kernel_variant(rank)["gradient_mfma"] == 1
+    print("mfma variant", variant, nW, "rank", rank, "tile", events, flush=True)
+    assert ctx.kernel_variant(rank)["gradient_mfma"] == (variant if variant == 1 or rank <= 2 else 0)
     f, d, df = ctx.fdf(prod, 0.1)
     wg = ctx.kernel_variant(rank)["workgroups"]
     f_ref = L.event_loglik(prod, 0.1)
@@ -126,17 +130,19 @@ def test_stage_gradient_on_mfma(gpu, case, events):
     p2 = prod.copy()
     p2[rank - 1, 1] *= -1
     assert ctx.log_likelihood(p2, 0.1) == f
-    ctx.set_stage_variant(events, False)
+    ctx.set_stage_variant(events, 0)
     fv, dv, dfv = ctx.fdf(prod, 0.1)
-    if ctx.kernel_variant(rank)["workgroups"] == wg:
+    if variant == 1 and ctx.kernel_variant(rank)["workgroups"] == wg:
         assert fv == f and dfv == df
+    assert abs(fv - f) <= 1e-12 * abs(f)
     assert np.max(np.abs(d - dv)) <= 1e-6 * np.max(np.abs(dv))
     ctx.close()
 
 
 def test_stage_gradient_on_mfma_rejected(gpu):
     """FP64 amplitudes, 16-event tiles and rows of more than 128 waves do not
-    take the MFMA gradient."""
+    take the MFMA gradient; the wave-tile variant needs 64-event tiles and rows
+    of at most 96 waves; variants other than 0-2 are invalid."""
     rng = np.random.default_rng(5)
     a = rng.normal(size=(100, 20)) + 1j * rng.normal(size=(100, 20))
     ctx = gpu.LikelihoodContext(0, 0)
@@ -148,7 +154,19 @@ def test_stage_gradient_on_mfma_rejected(gpu):
     ctx.load(a, (4, 16))
     with pytest.raises(gpu.RpwaError):
         ctx.set_stage_variant(16, True)
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(32, 2)
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(64, 3)
     ctx.set_stage_variant(32, True)
+    ctx.set_stage_variant(64, 2)
+    ctx.close()
+    c = rng.normal(size=(100, 100)) + 1j * rng.normal(size=(100, 100))
+    ctx = gpu.LikelihoodContext(0, 1)
+    ctx.load(c, (2, 98))  # 50 units: a 64-event tile exceeds the staging loads, 32 events fit
+    ctx.set_stage_variant(32, 1)
+    with pytest.raises(gpu.RpwaError):
+        ctx.set_stage_variant(64, 2)
     ctx.close()
     b = rng.normal(size=(100, 130)) + 1j * rng.normal(size=(100, 130))
     ctx = gpu.LikelihoodContext(0, 1)

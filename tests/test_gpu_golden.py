@@ -130,7 +130,7 @@ def test_every_staged_tile_size(gpu, nEvents):
     for prec, tol_f, tol_d in ((0, 1e-12, 1e-11), (1, 1e-6, 1e-5)):
         ctx = gpu.LikelihoodContext(0, prec)
         ctx.load(amps, nW)
-        variants = [(e, False) for e in (64, 32, 16)] + ([(64, True), (32, True)] if prec == 1 else [])
+        variants = [(e, 0) for e in (64, 32, 16)] + ([(64, 1), (32, 1), (64, 2)] if prec == 1 else [])
         for events, gm in variants:
             ctx.set_stage_variant(events, gm)
             print("staged tile", events, "mfma gradient", gm, "events", nEvents, "precision", prec, flush=True)

@@ -153,16 +153,16 @@ def test_steady_state_every_variant(gpu, long_list, precision):
     ctx = gpu.LikelihoodContext(0, precision)
     ctx.load(amps, nW)
     variants = [("stage", e) for e in (64, 32, 16)] + [("lanes", l) for l in (8, 16, 32, 64)]
-    variants += [("stage_mfma", e) for e in (64, 32)]
+    variants += [("stage_mfma", e) for e in (64, 32)] + [("stage_wave", 64)]
     for kind, v in variants:
         try:
             if kind == "lanes":
                 ctx.set_tile(v)
             else:
-                ctx.set_stage_variant(v, kind == "stage_mfma")
+                ctx.set_stage_variant(v, {"stage": 0, "stage_mfma": 1, "stage_wave": 2}[kind])
         except gpu.RpwaError as e:
             assert e.code == gpu.RPWA_ERR_UNSUPPORTED
-            assert kind != "stage_mfma" or precision == 0
+            assert kind not in ("stage_mfma", "stage_wave") or precision == 0
             continue
         f, d, df = ctx.fdf(prod, 0.3)
         info = ctx.info()
