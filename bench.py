@@ -132,6 +132,12 @@ def c0_leg(args, log):
 def kernel_label(info, precision, rank):
     """Name of the event-kernel variant the context's autotuner kept."""
     if info["lanes"] == 1:
+        if info.get("stage_variant") == 2:
+            return ("wave_tile_kernel<%s, rank %d, %d events per tile> (LDS-staged; each wave: S of 16 events on "
+                    "FP64 MFMA, sigma, gradient on FP32 MFMA)" % (precision, rank, info["slots"]))
+        if info.get("stage_variant") == 1:
+            return ("stage_kernel<%s, rank %d, %d events per tile> (LDS-staged, gradient on FP32 MFMA)"
+                    % (precision, rank, info["slots"]))
         return "stage_kernel<%s, rank %d, %d events per tile> (LDS-staged, fused lnL + gradient)" % (
             precision, rank, info["slots"])
     return "event_kernel<%s, %d lanes, %d slots, rank %d> (register tiles, fused lnL + gradient)" % (
@@ -265,6 +271,7 @@ def main():
         el = time.perf_counter() - t
         ev, red, n = ctx.kernel_times()
         info = ctx.info()
+        info["stage_variant"] = ctx.kernel_variant(rank)["gradient_mfma"]
         rate = args.steps / el
         k_avg = ev / max(n, 1)
         log("%s: %.1f evals/s (%.3f ms per FdF; event kernel %.3f ms, reduce %.3f ms; tile %d lanes x %d slots, "
