@@ -204,6 +204,7 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
         spectral = np.empty_like(mixed)
         spectral[:, 0] = init.predict(moments, nev, spectralOnly=True)
         spectral[:, 1:] = rnd[:, :nMixed - 1]
+        startValS, _ = bfS.eval(spectral)  # -lnL at the spectral start (the selection arm compares it)
         t = time.perf_counter()
         parS, valS, stS, neS = bfS.fit(startPar=spectral)
         tS += time.perf_counter() - t
@@ -212,12 +213,20 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
             sR = starts_needed(valR[k], best)
             sM = starts_needed(valM[k], best)
             sS = starts_needed(valS[k], best)
+            # selection arm: the network's start or the spectral one, whichever has the lower -lnL
+            # (one more evaluation), then the same random starts
+            useMlp = startVal[k][0] <= startValS[k][0]
+            sSel = sM if useMlp else sS
+            eSel = 1 + int((neM[k][:min(sM, nMixed)] if useMlp else neS[k][:min(sS, nMixed)]).sum())
             rows.append(dict(events=nev[k], best=best, startsRandom=sR, startsMixed=sM, startsSpectral=sS,
+                             startsSelected=sSel, selectedMlp=bool(useMlp), evalsSelected=eSel,
+                             selectedReaches=bool((valM[k][0] if useMlp else valS[k][0]) <= best + 0.5),
                              evalsRandom=int(neR[k][:min(sR, nRandom)].sum()),
                              evalsMixed=int(neM[k][:min(sM, nMixed)].sum()),
                              evalsSpectral=int(neS[k][:min(sS, nMixed)].sum()),
                              mlpValue=float(valM[k][0]), mlpEvals=int(neM[k][0]),
                              mlpStartGap=float(startVal[k][0] - best),
+                             spectralStartGap=float(startValS[k][0] - best),
                              randomStartGap=float(np.median(startVal[k][1:] - best)),
                              randomEvalsMean=float(neR[k].mean()), randomWithin=int(np.sum(valR[k] <= best + 0.5)),
                              spectralReaches=bool(valS[k][0] <= best + 0.5), spectralEvals=int(neS[k][0])))
@@ -230,8 +239,16 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
     eR = np.array([r["evalsRandom"] for r in rows])
     eM = np.array([r["evalsMixed"] for r in rows])
     eS = np.array([r["evalsSpectral"] for r in rows])
+    sSel = np.array([r["startsSelected"] for r in rows])
+    eSel = np.array([r["evalsSelected"] for r in rows])
     return {
         "bins": len(rows),
+        "selection_arm": {"note": "per bin the network's or the spectral start, whichever has the lower -lnL "
+                                  "(+1 evaluation), then the same random starts",
+                          "network_chosen": float(np.mean([r["selectedMlp"] for r in rows])),
+                          "starts_needed": float(sSel.mean()),
+                          "first_start_reaches_best": float(np.mean([r["selectedReaches"] for r in rows])),
+                          "evaluations_needed": float(eSel.mean())},
         "starts_needed": {"random": float(sR.mean()), "mlp_plus_random": float(sM.mean()),
                           "spectral_plus_random": float(sS.mean()), "ratio": float(sR.mean() / sM.mean())},
         "reached_within_budget": {"random_%d" % nRandom: float(np.mean(sR <= nRandom)),
@@ -240,6 +257,7 @@ def evaluate(init, seed, sizes, W, nRandom=64, nMixed=8, fitSeed=77, mcFactor=5,
         "mlp_start_reaches_best": float(np.mean(sM == 1)),
         "first_random_start_reaches_best": float(np.mean(sR == 1)),
         "start_gap_median": {"mlp": float(np.median([r["mlpStartGap"] for r in rows])),
+                             "spectral": float(np.median([r["spectralStartGap"] for r in rows])),
                              "random": float(np.median([r["randomStartGap"] for r in rows]))},
         "evaluations_needed": {"random": float(eR.mean()), "mlp_plus_random": float(eM.mean()),
                                "spectral_plus_random": float(eS.mean()),

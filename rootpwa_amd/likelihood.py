@@ -443,24 +443,35 @@ class pwaLikelihood:
         assert p.shape == ((n if n is not None else self.nmbPars()),)
         return p
 
+    def _eval_buffers(self, par):
+        """The parameters copied into a buffer of this object whose C pointer is
+        kept (converting a numpy array to a ctypes pointer costs microseconds,
+        as much as an evaluation at small sizes), and the gradient buffer."""
+        b = getattr(self, "_evalBufs", None)
+        n = getattr(self, "_nPars", None)
+        if b is None or n is None or b[0].size != n:
+            n = n if n is not None else self.nmbPars()
+            pb, gb = np.zeros(n), np.zeros(n)
+            b = self._evalBufs = (pb, gb, _d(pb), _d(gb))
+        np.copyto(b[0], self._par(par))
+        return b
+
     def DoEval(self, par):
-        p = self._par(par)
+        pb, _, pp, _ = self._eval_buffers(par)
         v = ctypes.c_double()
-        _call("rpwa_pwalik_doeval", self.h, _d(p), ctypes.byref(v))
+        _call("rpwa_pwalik_doeval", self.h, pp, ctypes.byref(v))
         return v.value
 
     def Gradient(self, par):
-        p = self._par(par)
-        g = np.zeros_like(p)
-        _call("rpwa_pwalik_gradient", self.h, _d(p), _d(g))
-        return g
+        pb, gb, pp, gp = self._eval_buffers(par)
+        _call("rpwa_pwalik_gradient", self.h, pp, gp)
+        return gb.copy()
 
     def FdF(self, par):
-        p = self._par(par)
-        g = np.zeros_like(p)
+        pb, gb, pp, gp = self._eval_buffers(par)
         v = ctypes.c_double()
-        _call("rpwa_pwalik_fdf", self.h, _d(p), ctypes.byref(v), _d(g))
-        return v.value, g
+        _call("rpwa_pwalik_fdf", self.h, pp, ctypes.byref(v), gp)
+        return v.value, gb.copy()
 
     def timeFdF(self, par, n):
         """n FdF calls at par from C++ (rpwa_pwalik_time_fdf): the native call rate
