@@ -588,6 +588,49 @@ __device__ __forceinline__ void load_plan(const StageArgs& a, int tid, uint32_t 
 	}
 }
 
+// Register-lean forms of the above for a kernel that holds two staging sets:
+// the tile offsets are recomputed from the thread index (load k of thread tid
+// is unit tid + NT k of the tile, or the tile's first unit past the tile) and
+// the LDS positions are kept as 16-bit pairs.
+template <int NT, int KL>
+__device__ __forceinline__ void fetch_tile_lean(const StageArgs& a, uint32_t t, uint32_t lastTile, int tid,
+                                                f32x4 (&buf)[KL])
+{
+	const uint64_t base = (uint64_t)t * a.tileUnits;
+	if (t < lastTile) {
+		const f32x4* tb = a.amps + base;
+#pragma unroll
+		for (int k = 0; k < KL; ++k) {
+			const uint32_t g = (uint32_t)tid + (uint32_t)NT * k;
+			buf[k] = __builtin_nontemporal_load(tb + (g < a.tileUnits ? g : 0u));
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < KL; ++k) {
+			const uint32_t g = (uint32_t)tid + (uint32_t)NT * k;
+			const uint64_t gg = base + (g < a.tileUnits ? g : 0u);
+			buf[k] = __builtin_nontemporal_load(a.amps + (gg < a.lastUnit ? gg : a.lastUnit));
+		}
+	}
+}
+
+template <int KL>
+__device__ __forceinline__ void pack_positions(const uint32_t (&ldsPos)[KL], uint32_t (&packed)[(KL + 1) / 2])
+{
+#pragma unroll
+	for (int k = 0; k < KL; k += 2)
+		packed[k / 2] = ldsPos[k] | ((k + 1 < KL ? ldsPos[k + 1] : 0u) << 16);
+}
+
+template <int KL>
+__device__ __forceinline__ void store_tile_packed(f32x4* stage, const uint32_t (&packed)[(KL + 1) / 2],
+                                                  const f32x4 (&buf)[KL])
+{
+#pragma unroll
+	for (int k = 0; k < KL; ++k)
+		stage[(packed[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = buf[k];
+}
+
 template <int KL>
 __device__ __forceinline__ void store_tile(f32x4* stage, const uint32_t (&ldsPos)[KL], const f32x4 (&buf)[KL])
 {
@@ -770,7 +813,7 @@ __device__ __forceinline__ void g_mfma_block(const char* rows, uint32_t rowBytes
 			GD[g][0][i] += (double)acc[g][i];
 }
 
-template <int R, int NG>
+template <int R, int NG, int KL>
 __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 {
 	constexpr int E = 64;
@@ -787,8 +830,13 @@ __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 	// then T' [unit][k][j] doubles (B of the S MFMA)
 	float* uw = reinterpret_cast<float*>(reinterpret_cast<char*>(smem) + a.stageBytes) + wid * (2 * 4 * EW);
 	double* tL = reinterpret_cast<double*>(reinterpret_cast<char*>(smem) + a.stageBytes + kWaves * 2 * 4 * EW * 4);
-	uint32_t ldsPos[kMaxLoads], ldOff[kMaxLoads];
-	load_plan<kThreads, kMaxLoads, E>(a, tid, ldsPos, ldOff);
+	// KL loads of 16 bytes per thread and tile (64 rows of at most 4 KL units)
+	uint32_t ldsPacked[(KL + 1) / 2];
+	{
+		uint32_t ldsPos[KL], ldOff[KL];
+		load_plan<kThreads, KL, E>(a, tid, ldsPos, ldOff);
+		pack_positions<KL>(ldsPos, ldsPacked);
+	}
 	{
 		typedef const __attribute__((address_space(4))) char cchar;
 		cdouble* tp = a.tInline ? (cdouble*)((cchar*)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(StageArgs, tin))
@@ -833,18 +881,14 @@ __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 	const int dEv = 4 * ((lane >> 2) & 3) + (lane >> 4);
 	const int dCol = lane & 3;
 
-	f32x4 buf[kMaxLoads];
+	// two staging register sets: the loads of the next two tiles are in flight
+	// while a tile is contracted (no fetch past the workgroup's last tile)
+	f32x4 bufA[KL], bufB[KL];
 	const uint32_t step = gridDim.x;
 	const uint32_t lastTile = a.nTiles ? a.nTiles - 1 : 0;
-	uint32_t tile = blockIdx.x;
-	if (tile < a.nTiles) {
-		fetch_tile(a, tile, lastTile, ldOff, buf);
-		const uint32_t nMine = (a.nTiles - 1 - tile) / step + 1;
-		for (uint32_t it = 0; it < nMine; ++it, tile += step) {
-			lds_barrier();  // every wave is done with the previous tile
-			store_tile(stage, ldsPos, buf);
-			fetch_tile(a, min(tile + step, lastTile), lastTile, ldOff, buf);
-			lds_barrier();
+	const uint32_t tile0 = blockIdx.x;
+	const uint32_t nMine = tile0 < a.nTiles ? (a.nTiles - 1 - tile0) / step + 1 : 0;
+	auto contract = [&](uint32_t tile) {
 			const char* sb = reinterpret_cast<const char*>(stage);
 			// ---- S of the wave's 16 events ----
 			const float* xa = reinterpret_cast<const float*>(sb + aRow);
@@ -879,6 +923,25 @@ __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			// ---- G of the wave's 16 events on the matrix cores ----
 			g_mfma_block<NG>(sb + (uint32_t)(wid * EW) * rowBytes, rowBytes, uw, aOff, bOff, GD);
+	};
+	if (nMine > 0)
+		fetch_tile_lean<kThreads>(a, tile0, lastTile, tid, bufA);
+	if (nMine > 1)
+		fetch_tile_lean<kThreads>(a, tile0 + step, lastTile, tid, bufB);
+	for (uint32_t it = 0; it < nMine; it += 2) {
+		lds_barrier();  // every wave is done with the previous tile
+		store_tile_packed<KL>(stage, ldsPacked, bufA);
+		if (it + 2 < nMine)
+			fetch_tile_lean<kThreads>(a, tile0 + (it + 2) * step, lastTile, tid, bufA);
+		lds_barrier();
+		contract(tile0 + it * step);
+		if (it + 1 < nMine) {
+			lds_barrier();
+			store_tile_packed<KL>(stage, ldsPacked, bufB);
+			if (it + 3 < nMine)
+				fetch_tile_lean<kThreads>(a, tile0 + (it + 3) * step, lastTile, tid, bufB);
+			lds_barrier();
+			contract(tile0 + (it + 1) * step);
 		}
 	}
 	write_partial_row_gm<R, E, NG, kThreads, true>(a, smem, tid, lane, wid, GD, mant, expo, wsum, lsum);
@@ -917,21 +980,35 @@ StageKernel pick_gm(int groups, int rank)
 	}
 }
 
-template <int R>
-StageKernel pick_wt(int groups)
+template <int R, int NG>
+StageKernel pick_wt_kl(int kl)
 {
+	// rows of 16 (NG - 1) + 1 .. 16 NG units: KL = ceil(rowUnits / 4) in 4 NG - 3 .. 4 NG
+	switch (kl - 4 * NG) {
+	case -3: return wave_tile_kernel<R, NG, 4 * NG - 3>;
+	case -2: return wave_tile_kernel<R, NG, 4 * NG - 2>;
+	case -1: return wave_tile_kernel<R, NG, 4 * NG - 1>;
+	default: return wave_tile_kernel<R, NG, 4 * NG>;
+	}
+}
+
+template <int R>
+StageKernel pick_wt(int rowUnits)
+{
+	const int groups = (rowUnits + kGmUnits - 1) / kGmUnits;
+	const int kl = (rowUnits + 3) / 4;
 	switch (groups) {
-	case 1: return wave_tile_kernel<R, 1>;
-	case 2: return wave_tile_kernel<R, 2>;
-	default: return wave_tile_kernel<R, 3>;
+	case 1: return pick_wt_kl<R, 1>(kl);
+	case 2: return pick_wt_kl<R, 2>(kl);
+	default: return pick_wt_kl<R, 3>(kl);
 	}
 }
 
 StageKernel pick_stage(const AmpLayout& lay, int events, int gm, int rank)
 {
 	if (gm == 2) {
-		const int groups = (int)((lay.rowLen * 8u / 16u + kGmUnits - 1) / kGmUnits);
-		return rank <= 1 ? pick_wt<1>(groups) : pick_wt<2>(groups);
+		const int rowUnits = (int)(lay.rowLen * 8u / 16u);
+		return rank <= 1 ? pick_wt<1>(rowUnits) : pick_wt<2>(rowUnits);
 	}
 	if (gm) {
 		const int groups = (int)((lay.rowLen * 8u / 16u + kGmUnits - 1) / kGmUnits);
