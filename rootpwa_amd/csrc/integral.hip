@@ -299,20 +299,62 @@ static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_
 	return C[(size_t)t * 256 + (r % 16) * 16 + (c % 16)];
 }
 
+// Per-device work area of the rank-k update (workgroup partial tiles, weight
+// sums, reduced tiles) and its stream, kept between calls and grown on demand:
+// a call then allocates nothing of its own on the device (an allocation and a
+// free of the ~35 MB partials, with the free's device synchronisation, cost
+// about as much as a tenth of the update at 5M x 88).  Calls on one device are
+// serialised by the mutex; the area lives until the process ends.
+struct SyrkWork {
+	std::mutex m;
+	hipStream_t s = nullptr;
+	double* part = nullptr;
+	double* wsum = nullptr;
+	double* red = nullptr;
+	size_t partCap = 0, wsumCap = 0, redCap = 0;
+};
+constexpr int kSyrkDevices = 64;
+
+static SyrkWork& syrk_work(int device)
+{
+	static SyrkWork work[kSyrkDevices];
+	return work[device];
+}
+
+static hipError_t syrk_grow(double*& buf, size_t& cap, size_t need)
+{
+	if (need <= cap)
+		return hipSuccess;
+	if (buf)
+		(void)hipFree(buf);
+	buf = nullptr;
+	cap = 0;
+	hipError_t e = hipMalloc(&buf, need * sizeof(double));
+	if (e == hipSuccess)
+		cap = need;
+	return e;
+}
+
 // Weighted real rank-k update C = sum_e wt_e x_e x_e^T of M rows of D doubles
 // (device or host memory); wt_e = w[e] (directWeights) or 1/w[e] or 1.
 // Returns the full symmetric D x D matrix and the sum of the weights.
 int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double* d_w, bool directWeights,
              const double* h_x, const double* h_w, std::vector<double>& full, double* wtot)
 {
+	if (device < 0 || device >= kSyrkDevices) {
+		set_error("rank-k update: device index out of range");
+		return RPWA_ERR_INVALID;
+	}
 	RPWA_HIP_CHECK(hipSetDevice(device));
 	IntegralPlan p;
 	int st = integral_plan(device, D, p);
 	if (st != RPWA_OK)
 		return st;
-	hipStream_t s;
-	RPWA_HIP_CHECK(hipStreamCreate(&s));
-	double *d_part = nullptr, *d_wsum = nullptr, *d_red = nullptr;
+	SyrkWork& work = syrk_work(device);
+	std::lock_guard<std::mutex> lock(work.m);
+	if (!work.s)
+		RPWA_HIP_CHECK(hipStreamCreateWithFlags(&work.s, hipStreamNonBlocking));
+	hipStream_t s = work.s;
 	double* d_stage[2] = {nullptr, nullptr};
 	double* d_wstage[2] = {nullptr, nullptr};
 	const size_t partN = (size_t)p.nWg * p.nTiles * 256;
@@ -320,9 +362,9 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	std::vector<double> C((size_t)p.nTiles * 256);
 	std::vector<double> wsums(p.nWg);
 	auto fail = [&](int code) {
-		hipFree(d_part); hipFree(d_wsum); hipFree(d_red);
+		if (code != RPWA_OK)
+			(void)hipStreamSynchronize(s);  // nothing of this call may still run on the work area
 		hipFree(d_stage[0]); hipFree(d_stage[1]); hipFree(d_wstage[0]); hipFree(d_wstage[1]);
-		hipStreamDestroy(s);
 		return code;
 	};
 #define INT_CHECK(expr)                                                               \
@@ -333,9 +375,12 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 			return fail(RPWA_ERR_HIP);                                                \
 		}                                                                             \
 	} while (0)
-	INT_CHECK(hipMalloc(&d_part, partN * sizeof(double)));
-	INT_CHECK(hipMalloc(&d_wsum, p.nWg * sizeof(double)));
-	INT_CHECK(hipMalloc(&d_red, (size_t)p.nTiles * 256 * sizeof(double)));
+	INT_CHECK(syrk_grow(work.part, work.partCap, partN));
+	INT_CHECK(syrk_grow(work.wsum, work.wsumCap, p.nWg));
+	INT_CHECK(syrk_grow(work.red, work.redCap, (size_t)p.nTiles * 256));
+	double* d_part = work.part;
+	double* d_wsum = work.wsum;
+	double* d_red = work.red;
 	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
 	// dynamic LDS beyond 64 KiB (the two chunk buffers at wide rows) is requested explicitly
 	if (p.lds + 32 * sizeof(double) > 64u * 1024u)
