@@ -127,8 +127,8 @@ int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProd
  * while timing is on (the sum of the event kernels' and the reduction's own
  * device time, without launch gaps; rpwa_lik_set_timing, or
  * RPWA_KERNEL_TIMING=1 in the environment; off by default: the events cost host
- * time on every evaluation, and with timing off an evaluation's completion is
- * read from a flag the reduction writes). */
+ * time on every evaluation).  Either way an evaluation's completion is read
+ * from flags the reduction writes into host memory. */
 int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds);
 int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx);
 /* Kernel timing on (enable != 0) or off for the following evaluations. */

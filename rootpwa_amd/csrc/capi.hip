@@ -493,7 +493,7 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 	// failed launch, a long kernel) fall back to the stream synchronisation,
 	// which also reports errors.
 	bool done = false;
-	if (!c->timing) {
+	{
 		const volatile unsigned long long* flag = c->h_flag;
 		const auto t0 = std::chrono::steady_clock::now();
 		uint32_t seen = 0;  // blocks 0..seen-1 are done
@@ -512,6 +512,10 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 	}
 	if (!done)
 		RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
+	// the timing events complete with the reduction's dispatch, just after its
+	// flags: waiting for the last one costs little once the flags are seen
+	if (c->timing)
+		RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
 	float ms = 0.f, ms1 = 0.f;
 	if (c->timing && hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
 	    hipEventElapsedTime(&ms1, c->evRed, c->ev1) == hipSuccess) {
