@@ -11,15 +11,18 @@
 // tiles of the upper triangle (tile row <= tile column), k = events.
 // A persistent grid of one 512-thread workgroup (8 waves) per CU walks event chunks
 // (grid-stride); each chunk is staged once through LDS and every wave issues
-// the MFMAs of its tiles from it.  Each workgroup writes its tile sums to a
-// partial row; partial rows are summed in fixed order (integral_reduce), and
-// the host assembles the complex matrix.
+// the MFMAs of its tiles from it.  Rows of up to 192 doubles (96 waves) take the
+// grouped kernel (tile map fixed at compile time, operand blocks read once per
+// step, pairs of waves splitting the steps); wider rows the generic one.  Each
+// workgroup writes its tile sums to a partial row; partial rows are summed in
+// fixed order (column_sum_kernel), and the host assembles the complex matrix.
 #include "internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace rpwa_amd {
@@ -201,6 +204,239 @@ __global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Grouped kernel for rows of up to 16 kGrpMaxBlocks doubles (NTB column blocks
+// of 16, row padded to Dp = 16 NTB).  The tile -> column-block map is fixed at
+// compile time, so a wave reads each 16-column operand block its tiles need once
+// per 4-event step and keeps it in registers (at most NTB LDS reads for up to
+// 20 MFMAs, instead of two reads per MFMA).  The upper-triangle tiles are split
+// into four contiguous groups of near-equal size, one per pair of waves
+// {g, g + 4}; the two waves of a pair take alternate 4-event steps of every
+// chunk, so no wave computes a dummy tile.  At the end the second wave of a
+// pair hands its sums to the first through LDS, in rounds that every wave runs
+// (same barriers), and the first writes the pair's sum (fixed order).
+constexpr int kGrpMaxBlocks = 12;
+constexpr int kLdsBytes = 160 * 1024;
+
+constexpr int tri_row(int nT, int t)
+{
+	int i = 0;
+	while (t >= nT - i) {
+		t -= nT - i;
+		++i;
+	}
+	return i;
+}
+constexpr int tri_col(int nT, int t) { return t - (tri_row(nT, t) * nT - tri_row(nT, t) * (tri_row(nT, t) - 1) / 2) + tri_row(nT, t); }
+constexpr int grp_begin(int nT, int g) { return g * (nT * (nT + 1) / 2) / 4; }
+constexpr int grp_size(int nT, int g) { return grp_begin(nT, g + 1) - grp_begin(nT, g); }
+constexpr int grp_max(int nT) { return (nT * (nT + 1) / 2 + 3) / 4; }
+// column blocks read by the tiles of group g: as A (rows) or as either operand
+constexpr uint32_t grp_blocks(int nT, int g, bool rowsOnly)
+{
+	uint32_t m = 0;
+	for (int t = grp_begin(nT, g); t < grp_begin(nT, g + 1); ++t) {
+		m |= 1u << tri_row(nT, t);
+		if (!rowsOnly)
+			m |= 1u << tri_col(nT, t);
+	}
+	return m;
+}
+// events per chunk: the largest multiple of 8 (up to 64) whose two LDS buffers,
+// [CH][Dp + 1] rows plus CH weights each, fit beside the 64 weight partials
+constexpr int grp_chunk(int nT)
+{
+	const int ch = (kLdsBytes - 64 * 8) / (2 * 8 * (16 * nT + 2)) / 8 * 8;
+	return ch < 64 ? ch : 64;
+}
+constexpr int grp_lds_doubles(int nT) { return 2 * grp_chunk(nT) * (16 * nT + 2); }
+// tiles per hand-over round: four groups' worth must fit in the chunk buffers
+constexpr int grp_round(int nT)
+{
+	const int r = grp_lds_doubles(nT) / (4 * 256);
+	return r < grp_max(nT) ? r : grp_max(nT);
+}
+
+template <int NT, int T>
+struct TileRC {
+	static constexpr int r = tri_row(NT, T);
+	static constexpr int c = tri_col(NT, T);
+};
+
+// one 4-event step of group G: read the group's operand blocks from the LDS row
+// xr (already offset by the lane's column), then one MFMA per tile
+template <int NTB, int G, bool WT, int... Q>
+__device__ __forceinline__ void grp_step(const double* __restrict__ xr, double wt, d4* acc,
+                                         std::integer_sequence<int, Q...>)
+{
+	constexpr int t0 = grp_begin(NTB, G);
+	constexpr uint32_t used = grp_blocks(NTB, G, false);
+	constexpr uint32_t rows = grp_blocks(NTB, G, true);
+	double v[NTB], va[NTB];
+#pragma unroll
+	for (int b = 0; b < NTB; ++b)
+		v[b] = (used >> b & 1) ? xr[16 * b] : 0.;
+#pragma unroll
+	for (int b = 0; b < NTB; ++b)
+		va[b] = (WT && (rows >> b & 1)) ? v[b] * wt : v[b];
+	((acc[Q] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[TileRC<NTB, t0 + Q>::r], v[TileRC<NTB, t0 + Q>::c], acc[Q],
+	                                                 0, 0, 0)),
+	 ...);
+}
+
+template <int NTB, bool WEIGHTED, int G>
+__device__ __forceinline__ void grp_body(const IntArgs& a, double* lds, double* wsh)
+{
+	constexpr int CH = grp_chunk(NTB);
+	// doubles per LDS row (16 NTB + 1) and per chunk buffer (rows, then weights); as
+	// run-time values the kernel measured 3% faster at 11 blocks than with them folded
+	const uint32_t STRIDE = a.ldsStride;
+	const uint32_t BUF = CH * STRIDE + CH;
+	constexpr int KL = (CH * 16 * NTB / 2 + kIntThreads - 1) / kIntThreads;  // double2 loads per thread
+	constexpr int NG = grp_size(NTB, G);
+	constexpr int kMaxG = grp_max(NTB);
+	constexpr int R = grp_round(NTB);
+	static_assert(CH % 8 == 0 && R >= 1, "chunk of whole step pairs, hand-over round of a tile at least");
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int h = tid >> 8;     // which wave of the pair (waves 0-3 or 4-7)
+	const int li = lane & 15;
+	const int lk = lane >> 4;
+
+	d4 acc[NG > 0 ? NG : 1];
+#pragma unroll
+	for (int q = 0; q < NG; ++q)
+		acc[q] = d4{0., 0., 0., 0.};
+
+	const uint32_t chunkD2 = CH * a.D / 2;
+	uint32_t ldsOff[KL];
+#pragma unroll
+	for (int l = 0; l < KL; ++l) {
+		const uint32_t idx = 2 * ((uint32_t)tid + (uint32_t)l * kIntThreads);
+		ldsOff[l] = (idx / a.D) * STRIDE + idx % a.D;
+	}
+	// padding columns (D..Dp) of both buffers stay zero
+	for (uint32_t i = tid; i < 2 * BUF; i += kIntThreads)
+		lds[i] = 0.;
+	__syncthreads();
+
+	double wacc = 0.;
+	const uint64_t nChunks = (a.M + CH - 1) / CH;
+	double2 stage[KL];
+	double wstage = 0.;
+	auto load_chunk = [&](uint64_t c) {
+		const uint64_t e0 = c * CH;
+		const uint64_t nEv = a.M - e0 < (uint64_t)CH ? a.M - e0 : (uint64_t)CH;
+		const uint32_t lim = (uint32_t)nEv * a.D / 2;
+		const double2* src = reinterpret_cast<const double2*>(a.x + e0 * a.D);
+#pragma unroll
+		for (int l = 0; l < KL; ++l) {
+			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
+			stage[l] = (i2 < lim) ? src[i2] : make_double2(0., 0.);
+		}
+		if (tid < CH) {
+			const uint64_t e = e0 + tid;
+			wstage = e < a.M ? (a.w ? (a.directWeights ? a.w[e] : 1. / a.w[e]) : 1.) : 0.;
+		}
+	};
+	auto store_chunk = [&](int buf) {
+		double* xs = lds + buf * BUF;
+#pragma unroll
+		for (int l = 0; l < KL; ++l) {
+			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
+			if (i2 < chunkD2) {
+				xs[ldsOff[l]] = stage[l].x;
+				xs[ldsOff[l] + 1] = stage[l].y;
+			}
+		}
+		if (tid < CH) {
+			xs[CH * STRIDE + tid] = wstage;
+			wacc += wstage;
+		}
+	};
+
+	uint64_t c = blockIdx.x;
+	int buf = 0;
+	if (c < nChunks) {
+		load_chunk(c);
+		store_chunk(0);
+	}
+	__syncthreads();
+	for (; c < nChunks; c += gridDim.x) {
+		const uint64_t next = c + gridDim.x;
+		if (next < nChunks)
+			load_chunk(next);  // in flight while the MFMAs below run
+		const double* xs = lds + buf * BUF;
+		const double* wts = xs + CH * STRIDE;
+		// not unrolled: unrolled, the steps' operand reads are hoisted together (more
+		// registers, 4% slower at 11 blocks)
+#pragma unroll 1
+		for (int kk = 0; kk < CH / 8; ++kk) {
+			const int row = (2 * kk + h) * 4 + lk;
+			const double wt = WEIGHTED ? wts[row] : 1.;
+			grp_step<NTB, G, WEIGHTED>(xs + row * STRIDE + li, wt, acc, std::make_integer_sequence<int, NG>{});
+		}
+		if (next < nChunks)
+			store_chunk(buf ^ 1);
+		__syncthreads();
+		buf ^= 1;
+	}
+
+	// hand-over: [group][tile of the round][register][lane] in the chunk buffers;
+	// v_mfma_f64_16x16x4_f64 leaves D[i][j] in lane j + 16 (i % 4), register i / 4
+	double* red = lds + G * R * 256;
+	double* out = a.partials + (size_t)blockIdx.x * a.nTiles * 256;
+#pragma unroll
+	for (int r0 = 0; r0 < kMaxG; r0 += R) {
+		if (h == 1) {
+#pragma unroll
+			for (int q = r0; q < NG && q < r0 + R; ++q)
+#pragma unroll
+				for (int v = 0; v < 4; ++v)
+					red[((q - r0) * 4 + v) * 64 + lane] = acc[q][v];
+		}
+		__syncthreads();
+		if (h == 0) {
+#pragma unroll
+			for (int q = r0; q < NG && q < r0 + R; ++q) {
+				double* o = out + (size_t)(grp_begin(NTB, G) + q) * 256;
+#pragma unroll
+				for (int v = 0; v < 4; ++v) {
+					const int i = lk + 4 * v;
+					double val = acc[q][v] + red[((q - r0) * 4 + v) * 64 + lane];
+					if (a.accumulate)
+						val += o[i * 16 + li];
+					o[i * 16 + li] = val;
+				}
+			}
+		}
+		__syncthreads();
+	}
+	if (tid < CH)
+		wsh[tid] = wacc;
+	__syncthreads();
+	if (a.wsum && tid == 0) {
+		double s = 0.;
+		for (int r = 0; r < CH; ++r)
+			s += wsh[r];
+		a.wsum[blockIdx.x] = a.accumulate ? a.wsum[blockIdx.x] + s : s;
+	}
+}
+
+template <int NTB, bool WEIGHTED>
+__global__ __launch_bounds__(kIntThreads) void integral_grp_kernel(IntArgs a)
+{
+	extern __shared__ double lds[];
+	__shared__ double wsh[64];
+	// wave-uniform branch: waves g and g + 4 run group g
+	switch ((threadIdx.x >> 6) & 3) {
+	case 0: grp_body<NTB, WEIGHTED, 0>(a, lds, wsh); break;
+	case 1: grp_body<NTB, WEIGHTED, 1>(a, lds, wsh); break;
+	case 2: grp_body<NTB, WEIGHTED, 2>(a, lds, wsh); break;
+	default: grp_body<NTB, WEIGHTED, 3>(a, lds, wsh); break;
+	}
+}
+
 // fixed-order column sums of a [nRows][nCols] array
 constexpr int kColElems = 64;
 constexpr int kColChunks = 16;
@@ -229,37 +465,6 @@ __global__ __launch_bounds__(kColElems* kColChunks) void column_sum_kernel(
 
 }  // namespace
 
-struct IntegralPlan {
-	uint32_t D, Dp, nT, nTiles, ldsStride, tilesPerPass;
-	int chunk;
-	int nWg;
-	size_t lds;
-};
-
-// D = real row length (2 x complex values per row)
-static int integral_plan(int device, uint32_t D, IntegralPlan& p)
-{
-	p.D = D;
-	p.Dp = (p.D + 15) / 16 * 16;
-	p.nT = p.Dp / 16;
-	p.nTiles = p.nT * (p.nT + 1) / 2;
-	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
-	p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
-	p.chunk = 32;
-	while (p.chunk > 4 && (size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread)
-		p.chunk /= 2;
-	if ((size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
-		set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / 8) +
-		          " complex values per row");
-		return RPWA_ERR_UNSUPPORTED;
-	}
-	p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
-	int nCu = 0;
-	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
-	p.nWg = nCu > 0 ? nCu : 1;
-	return RPWA_OK;
-}
-
 typedef void (*IntKernel)(IntArgs);
 template <int CH, bool WT>
 static IntKernel pick_int_nt(uint32_t per)
@@ -281,10 +486,66 @@ static IntKernel pick_int_chunk(uint32_t per, int chunk)
 	return pick_int_nt<4, WT>(per);
 }
 
-static IntKernel pick_int_kernel(uint32_t nTiles, int chunk, bool weighted)
+template <bool WT>
+static IntKernel pick_grp(uint32_t nT)
 {
-	const uint32_t per = (nTiles + kIntWaves - 1) / kIntWaves;
-	return weighted ? pick_int_chunk<true>(per, chunk) : pick_int_chunk<false>(per, chunk);
+	switch (nT) {
+	case 1: return integral_grp_kernel<1, WT>;
+	case 2: return integral_grp_kernel<2, WT>;
+	case 3: return integral_grp_kernel<3, WT>;
+	case 4: return integral_grp_kernel<4, WT>;
+	case 5: return integral_grp_kernel<5, WT>;
+	case 6: return integral_grp_kernel<6, WT>;
+	case 7: return integral_grp_kernel<7, WT>;
+	case 8: return integral_grp_kernel<8, WT>;
+	case 9: return integral_grp_kernel<9, WT>;
+	case 10: return integral_grp_kernel<10, WT>;
+	case 11: return integral_grp_kernel<11, WT>;
+	case 12: return integral_grp_kernel<12, WT>;
+	default: return nullptr;
+	}
+}
+static int grp_chunk_of(uint32_t nT) { return grp_chunk((int)nT); }
+
+struct IntegralPlan {
+	uint32_t D, Dp, nT, nTiles, ldsStride, tilesPerPass;
+	int chunk;
+	int nWg;
+	size_t lds;
+	IntKernel kern;  // grouped kernel (one pass) or generic kernel (passes of tilesPerPass tiles)
+};
+
+// D = real row length (2 x complex values per row)
+static int integral_plan(int device, uint32_t D, bool weighted, IntegralPlan& p)
+{
+	p.D = D;
+	p.Dp = (p.D + 15) / 16 * 16;
+	p.nT = p.Dp / 16;
+	p.nTiles = p.nT * (p.nT + 1) / 2;
+	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
+	p.kern = weighted ? pick_grp<true>(p.nT) : pick_grp<false>(p.nT);
+	if (p.kern) {
+		p.tilesPerPass = p.nTiles;
+		p.chunk = grp_chunk_of(p.nT);
+		p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
+	} else {
+		p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
+		p.chunk = 32;
+		while (p.chunk > 4 && (size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread)
+			p.chunk /= 2;
+		if ((size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
+			set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / 8) +
+			          " complex values per row");
+			return RPWA_ERR_UNSUPPORTED;
+		}
+		p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
+		const uint32_t per = (p.tilesPerPass + kIntWaves - 1) / kIntWaves;
+		p.kern = weighted ? pick_int_chunk<true>(per, p.chunk) : pick_int_chunk<false>(per, p.chunk);
+	}
+	int nCu = 0;
+	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
+	p.nWg = nCu > 0 ? nCu : 1;
+	return RPWA_OK;
 }
 
 static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_t r, uint32_t c)
@@ -346,8 +607,9 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 		return RPWA_ERR_INVALID;
 	}
 	RPWA_HIP_CHECK(hipSetDevice(device));
+	const bool weighted = (d_w != nullptr) || (h_w != nullptr);
 	IntegralPlan p;
-	int st = integral_plan(device, D, p);
+	int st = integral_plan(device, D, weighted, p);
 	if (st != RPWA_OK)
 		return st;
 	SyrkWork& work = syrk_work(device);
@@ -358,7 +620,6 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	double* d_stage[2] = {nullptr, nullptr};
 	double* d_wstage[2] = {nullptr, nullptr};
 	const size_t partN = (size_t)p.nWg * p.nTiles * 256;
-	const bool weighted = (d_w != nullptr) || (h_w != nullptr);
 	std::vector<double> C((size_t)p.nTiles * 256);
 	std::vector<double> wsums(p.nWg);
 	auto fail = [&](int code) {
@@ -381,9 +642,9 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	double* d_part = work.part;
 	double* d_wsum = work.wsum;
 	double* d_red = work.red;
-	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
-	// dynamic LDS beyond 64 KiB (the two chunk buffers at wide rows) is requested explicitly
-	if (p.lds + 32 * sizeof(double) > 64u * 1024u)
+	IntKernel k = p.kern;
+	// dynamic LDS beyond 64 KiB (the two chunk buffers) is requested explicitly
+	if (p.lds + 64 * sizeof(double) > 64u * 1024u)
 		INT_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
 		                              (int)p.lds));
 	// every pass re-reads the rows for its own set of output tiles
@@ -636,13 +897,13 @@ static int integral_noalloc(int device, const double* d_amps, uint64_t M, uint32
 		return RPWA_ERR_HIP;
 	}
 	double* const base = g_naPtr[device];
+	const bool weighted = d_w != nullptr;
 	IntegralPlan p;
-	if (int st = integral_plan(device, 2 * W, p))
+	if (int st = integral_plan(device, 2 * W, weighted, p))
 		return st;
 	p.nWg = std::min(p.nWg, kNaWg);
-	const bool weighted = d_w != nullptr;
-	IntKernel k = pick_int_kernel(p.tilesPerPass, p.chunk, weighted);
-	if (p.lds + 32 * sizeof(double) > 64u * 1024u)
+	IntKernel k = p.kern;
+	if (p.lds + 64 * sizeof(double) > 64u * 1024u)
 		RPWA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
 		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
 	IntArgs a;

@@ -206,6 +206,22 @@ def test_integral_matches_oracle(gpu):
         assert H.rel_err(gpu.integral(a, w), O.integral(a, w)) <= 1e-12, W
 
 
+def test_integral_several_chunks_per_workgroup(gpu):
+    """Every workgroup of the persistent grid walks at least three event chunks (up to 64 events each), so
+    the staged double buffer, the accumulation over chunks and the ragged last chunk are all exercised:
+    5 column blocks (W=40), 11 (W=88), 12 (W=96: the pair hand-over in two rounds), with and without weights."""
+    rng = np.random.default_rng(21)
+    n_wg = gpu.device_info(0)["cus"]  # one workgroup per CU
+    for W in (40, 88, 96):
+        M = 3 * 64 * n_wg + 17
+        print("integral W", W, "M", M, flush=True)
+        a = rng.normal(size=(M, W)) + 1j * rng.normal(size=(M, W))
+        a *= np.linspace(0.3, 1.7, W)[None, :]
+        assert H.rel_err(gpu.integral(a), O.integral(a)) <= 1e-12, W
+        w = rng.uniform(0.5, 2.0, size=M)
+        assert H.rel_err(gpu.integral(a, w), O.integral(a, w)) <= 1e-12, W
+
+
 def test_integral_incoherent_sub_amplitudes(gpu):
     """ampIntegralMatrix with two incoherent sub-amplitudes per wave
     (ampIntegralMatrix.cc:432-466), host and device input."""
