@@ -13,9 +13,10 @@
 // (grid-stride); each chunk is staged once through LDS and every wave issues
 // the MFMAs of its tiles from it.  Rows of up to 192 doubles (96 waves) take the
 // grouped kernel (tile map fixed at compile time, operand blocks read once per
-// step, pairs of waves splitting the steps); wider rows the generic one.  Each
-// workgroup writes its tile sums to a partial row; partial rows are summed in
-// fixed order (column_sum_kernel), and the host assembles the complex matrix.
+// step, pairs of waves splitting the steps); wider rows take windowed passes of
+// the same scheme over column windows of 8 blocks.  Each workgroup writes its
+// tile sums to a partial row; partial rows are summed in fixed order
+// (column_sum_kernel), and the host assembles the complex matrix.
 #include "internal.h"
 
 #include <algorithm>
@@ -31,9 +32,6 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIntThreads = 512;
-constexpr int kIntWaves = 8;
-constexpr int kMaxTilesPerWave = 12;  // accumulators of 12 tiles per wave stay in registers at 2 waves per SIMD
-constexpr int kMaxLoadsPerThread = 8;  // double2 loads per thread and chunk: CH * D / 2 <= 4096
 
 struct IntArgs {
 	const double* x;       // [M][D] row-major (interleaved complex rows)
@@ -44,165 +42,16 @@ struct IntArgs {
 	uint32_t ldsStride;    // doubles per LDS row
 	uint32_t nT;           // Dp / 16
 	uint32_t nTiles;       // nT (nT + 1) / 2
-	uint32_t tileBase;     // first tile of this pass
-	uint32_t tileEnd;      // one past the last tile of this pass
+	uint32_t tileBase;     // windowed kernel: the pass index (the first pass writes the weight sums)
+	uint32_t winCol0;      // windowed kernel: first column (double) of the row window
+	uint32_t winCol1;      // windowed kernel: first column of the column window (off-diagonal pass)
+	uint32_t winI0;        // windowed kernel: first column block of the row window
+	uint32_t winJ0;        // windowed kernel: first column block of the column window
 	double* partials;      // [nWg][nTiles][256]
 	double* wsum;          // [nWg] sum of weights (1/w), or null
 	int accumulate;        // add to the existing partials
 	int directWeights;     // weight = w[e] instead of 1/w[e]
 };
-
-__device__ __forceinline__ void tile_of(uint32_t t, uint32_t nT, uint32_t& I, uint32_t& J)
-{
-	// tiles enumerated row by row over the upper triangle
-	uint32_t i = 0, rem = t;
-	while (rem >= nT - i) {
-		rem -= nT - i;
-		++i;
-	}
-	I = i;
-	J = i + rem;
-}
-
-// NT tiles per wave: wave w owns the contiguous block [w*NT, (w+1)*NT) of the
-// pass; slots past the end of the pass compute a dummy tile that is discarded,
-// so the MFMA loop has no data-dependent exits and the accumulators stay in
-// registers.  WEIGHTED = false: every weight is 1 (no multiply per MFMA operand).
-template <int NT, int kChunk, bool WEIGHTED>
-__global__ __launch_bounds__(kIntThreads) void integral_kernel(IntArgs a)
-{
-	extern __shared__ double lds[];   // 2 x ([kChunk][ldsStride] + weights[kChunk])
-	const uint32_t bufDoubles = kChunk * a.ldsStride + kChunk;
-	const int tid = threadIdx.x;
-	const int lane = tid & 63;
-	const int wid = tid >> 6;
-	const int li = lane & 15;   // row/column within the tile
-	const int lk = lane >> 4;   // k index within a 4-event step
-
-	uint32_t tI[NT], tJ[NT];
-	bool keep[NT];
-	d4 acc[NT];
-#pragma unroll
-	for (int q = 0; q < NT; ++q) {
-		const uint32_t t = a.tileBase + (uint32_t)(wid * NT + q);
-		uint32_t I = 0, J = 0;
-		keep[q] = t < a.tileEnd;
-		if (keep[q])
-			tile_of(t, a.nT, I, J);
-		tI[q] = I * 16 + li;
-		tJ[q] = J * 16 + li;
-		acc[q] = d4{0., 0., 0., 0.};
-	}
-
-	// staging geometry: thread loads double2 number tid + 256 l of the chunk
-	const uint32_t chunkD2 = kChunk * a.D / 2;
-	uint32_t ldsOff[kMaxLoadsPerThread];
-#pragma unroll
-	for (int l = 0; l < kMaxLoadsPerThread; ++l) {
-		const uint32_t idx = 2 * ((uint32_t)tid + (uint32_t)l * kIntThreads);
-		ldsOff[l] = (idx / a.D) * a.ldsStride + idx % a.D;
-	}
-	// zero both buffers once: padding columns (D..Dp) stay zero
-	for (uint32_t i = tid; i < 2 * bufDoubles; i += kIntThreads)
-		lds[i] = 0.;
-	__syncthreads();
-
-	double wacc = 0.;
-	const uint64_t nChunks = (a.M + kChunk - 1) / kChunk;
-	double2 stage[kMaxLoadsPerThread];
-	double wstage = 0.;
-	auto load_chunk = [&](uint64_t c) {
-		const uint64_t e0 = c * kChunk;
-		const uint64_t nEv = a.M - e0 < (uint64_t)kChunk ? a.M - e0 : (uint64_t)kChunk;
-		const uint32_t lim = (uint32_t)nEv * a.D / 2;
-		const double2* src = reinterpret_cast<const double2*>(a.x + e0 * a.D);
-#pragma unroll
-		for (int l = 0; l < kMaxLoadsPerThread; ++l) {
-			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
-			stage[l] = (i2 < lim) ? src[i2] : make_double2(0., 0.);
-		}
-		if (tid < kChunk) {
-			const uint64_t e = e0 + tid;
-			wstage = e < a.M ? (a.w ? (a.directWeights ? a.w[e] : 1. / a.w[e]) : 1.) : 0.;
-		}
-	};
-	auto store_chunk = [&](int buf) {
-		double* xs = lds + (size_t)buf * bufDoubles;
-#pragma unroll
-		for (int l = 0; l < kMaxLoadsPerThread; ++l) {
-			const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
-			if (i2 < chunkD2) {
-				xs[ldsOff[l]] = stage[l].x;
-				xs[ldsOff[l] + 1] = stage[l].y;
-			}
-		}
-		if (tid < kChunk) {
-			xs[kChunk * a.ldsStride + tid] = wstage;
-			wacc += wstage;
-		}
-	};
-
-	uint64_t c = blockIdx.x;
-	int buf = 0;
-	if (c < nChunks) {
-		load_chunk(c);
-		store_chunk(0);
-	}
-	__syncthreads();
-	for (; c < nChunks; c += gridDim.x) {
-		const uint64_t next = c + gridDim.x;
-		if (next < nChunks)
-			load_chunk(next);  // in flight while the MFMAs below run
-		const double* xs = lds + (size_t)buf * bufDoubles;
-		const double* wts = xs + kChunk * a.ldsStride;
-#pragma unroll
-		for (int ks = 0; ks < kChunk / 4; ++ks) {
-			const int row = ks * 4 + lk;
-			const double wt = WEIGHTED ? wts[row] : 1.;
-			const double* xr = xs + (size_t)row * a.ldsStride;
-#pragma unroll
-			for (int q = 0; q < NT; ++q) {
-				const double av = WEIGHTED ? xr[tI[q]] * wt : xr[tI[q]];
-				const double bv = xr[tJ[q]];
-				acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
-			}
-		}
-		if (next < nChunks)
-			store_chunk(buf ^ 1);
-		__syncthreads();
-		buf ^= 1;
-	}
-
-	// v_mfma_f64_16x16x4_f64 result layout (checked on gfx950):
-	// D[i][j] sits in lane j + 16 * (i % 4), register i / 4
-	double* out = a.partials + (size_t)blockIdx.x * a.nTiles * 256;
-#pragma unroll
-	for (int q = 0; q < NT; ++q) {
-		if (!keep[q])
-			continue;
-		const uint32_t t = a.tileBase + (uint32_t)(wid * NT + q);
-		double* o = out + (size_t)t * 256;
-#pragma unroll
-		for (int v = 0; v < 4; ++v) {
-			const int i = lk + 4 * v;
-			const int j = li;
-			double val = acc[q][v];
-			if (a.accumulate)
-				val += o[i * 16 + j];
-			o[i * 16 + j] = val;
-		}
-	}
-	__shared__ double wsh[32];
-	if (tid < kChunk)
-		wsh[tid] = wacc;
-	__syncthreads();
-	if (a.wsum && a.tileBase == 0 && tid == 0) {
-		double s = 0.;
-		for (int r = 0; r < kChunk; ++r)
-			s += wsh[r];
-		a.wsum[blockIdx.x] = a.accumulate ? a.wsum[blockIdx.x] + s : s;
-	}
-}
 
 // ---------------------------------------------------------------------------
 // Grouped kernel for rows of up to 16 kGrpMaxBlocks doubles (NTB column blocks
@@ -437,6 +286,221 @@ __global__ __launch_bounds__(kIntThreads) void integral_grp_kernel(IntArgs a)
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Windowed passes for rows wider than 16 kGrpMaxBlocks doubles (the Hessian's
+// derivative rows): the column blocks are cut into windows of kWin; a pass
+// stages only its windows' columns of every row (one window for a diagonal
+// pass, the kWin (kWin + 1) / 2 tiles of its triangle; two windows for an
+// off-diagonal pass, the kWin x kWin tiles between them) and computes its tiles
+// with the grouped kernel's scheme (compile-time tile map, operand blocks read
+// once per step, pairs of waves splitting the steps).  Blocks past the row
+// (the last window's padding) are staged as zeros and their tiles not written.
+constexpr int kWin = 8;
+
+constexpr int win_blocks_of(bool rect) { return rect ? 2 * kWin : kWin; }
+constexpr int win_tiles(bool rect) { return rect ? kWin * kWin : kWin * (kWin + 1) / 2; }
+constexpr int win_row(bool rect, int t) { return rect ? t / kWin : tri_row(kWin, t); }
+constexpr int win_col(bool rect, int t) { return rect ? kWin + t % kWin : tri_col(kWin, t); }
+constexpr int win_begin(bool rect, int g) { return g * win_tiles(rect) / 4; }
+constexpr int win_size(bool rect, int g) { return win_begin(rect, g + 1) - win_begin(rect, g); }
+constexpr int win_max(bool rect) { return (win_tiles(rect) + 3) / 4; }
+constexpr uint32_t win_blocks(bool rect, int g, bool rowsOnly)
+{
+	uint32_t m = 0;
+	for (int t = win_begin(rect, g); t < win_begin(rect, g + 1); ++t) {
+		m |= 1u << win_row(rect, t);
+		if (!rowsOnly)
+			m |= 1u << win_col(rect, t);
+	}
+	return m;
+}
+constexpr int win_chunk(bool rect)
+{
+	const int ch = (kLdsBytes - 64 * 8) / (2 * 8 * (16 * win_blocks_of(rect) + 2)) / 8 * 8;
+	return ch < 64 ? ch : 64;
+}
+constexpr int win_lds_doubles(bool rect) { return 2 * win_chunk(rect) * (16 * win_blocks_of(rect) + 2); }
+
+template <bool RECT, int T>
+struct WinRC {
+	static constexpr int r = win_row(RECT, T);
+	static constexpr int c = win_col(RECT, T);
+};
+
+template <bool RECT, int G, bool WT, int... Q>
+__device__ __forceinline__ void win_step(const double* __restrict__ xr, double wt, d4* acc,
+                                         std::integer_sequence<int, Q...>)
+{
+	constexpr int NB = win_blocks_of(RECT);
+	constexpr int t0 = win_begin(RECT, G);
+	constexpr uint32_t used = win_blocks(RECT, G, false);
+	constexpr uint32_t rows = win_blocks(RECT, G, true);
+	double v[NB], va[NB];
+#pragma unroll
+	for (int b = 0; b < NB; ++b)
+		v[b] = (used >> b & 1) ? xr[16 * b] : 0.;
+#pragma unroll
+	for (int b = 0; b < NB; ++b)
+		va[b] = (WT && (rows >> b & 1)) ? v[b] * wt : v[b];
+	((acc[Q] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[WinRC<RECT, t0 + Q>::r], v[WinRC<RECT, t0 + Q>::c], acc[Q],
+	                                                 0, 0, 0)),
+	 ...);
+}
+
+template <bool RECT, bool WEIGHTED, int G>
+__device__ __forceinline__ void win_body(const IntArgs& a, double* lds, double* wsh)
+{
+	constexpr int CH = win_chunk(RECT);
+	constexpr int WS = 16 * win_blocks_of(RECT);       // staged doubles per row
+	constexpr int W0 = 16 * kWin;                      // doubles of the row window
+	constexpr int KL = CH * WS / 2 / kIntThreads;      // double2 loads per thread (exact)
+	constexpr int NG = win_size(RECT, G);
+	constexpr int kMaxG = win_max(RECT);
+	static_assert(CH * WS / 2 == KL * kIntThreads && CH % 8 == 0, "chunk of whole loads and step pairs");
+	static_assert(win_lds_doubles(RECT) >= 4 * kMaxG * 256, "hand-over in one round");
+	// LDS row stride (WS + 1) as a run-time value, as in the grouped kernel
+	const uint32_t STRIDE = a.ldsStride;
+	const uint32_t BUF = CH * STRIDE + CH;
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int h = tid >> 8;
+	const int li = lane & 15;
+	const int lk = lane >> 4;
+
+	d4 acc[NG];
+#pragma unroll
+	for (int q = 0; q < NG; ++q)
+		acc[q] = d4{0., 0., 0., 0.};
+
+	// staging geometry: double2 number tid + 512 l of the chunk's staged block is
+	// column k of event e of the block; its source column g in the row
+	uint32_t ldsOff[KL], srcOff[KL], srcEv[KL];
+	bool colOk[KL];
+#pragma unroll
+	for (int l = 0; l < KL; ++l) {
+		const uint32_t i2 = (uint32_t)tid + (uint32_t)l * kIntThreads;
+		const uint32_t e = i2 / (WS / 2), k = 2 * (i2 % (WS / 2));
+		const uint32_t g = k < (uint32_t)W0 ? a.winCol0 + k : a.winCol1 + (k - W0);
+		ldsOff[l] = e * STRIDE + k;
+		srcOff[l] = e * a.D + g;
+		srcEv[l] = e;
+		colOk[l] = g < a.D;
+	}
+	for (uint32_t i = tid; i < 2 * BUF; i += kIntThreads)
+		lds[i] = 0.;
+	__syncthreads();
+
+	double wacc = 0.;
+	const uint64_t nChunks = (a.M + CH - 1) / CH;
+	double2 stage[KL];
+	double wstage = 0.;
+	auto load_chunk = [&](uint64_t c) {
+		const uint64_t e0 = c * CH;
+		const uint64_t nEv = a.M - e0 < (uint64_t)CH ? a.M - e0 : (uint64_t)CH;
+		const double* src = a.x + e0 * a.D;
+#pragma unroll
+		for (int l = 0; l < KL; ++l)
+			stage[l] = (colOk[l] && srcEv[l] < (uint32_t)nEv) ? *reinterpret_cast<const double2*>(src + srcOff[l])
+			                                                  : make_double2(0., 0.);
+		if (tid < CH) {
+			const uint64_t e = e0 + tid;
+			wstage = e < a.M ? (a.w ? (a.directWeights ? a.w[e] : 1. / a.w[e]) : 1.) : 0.;
+		}
+	};
+	auto store_chunk = [&](int buf) {
+		double* xs = lds + buf * BUF;
+#pragma unroll
+		for (int l = 0; l < KL; ++l) {
+			xs[ldsOff[l]] = stage[l].x;
+			xs[ldsOff[l] + 1] = stage[l].y;
+		}
+		if (tid < CH) {
+			xs[CH * STRIDE + tid] = wstage;
+			wacc += wstage;
+		}
+	};
+
+	uint64_t c = blockIdx.x;
+	int buf = 0;
+	if (c < nChunks) {
+		load_chunk(c);
+		store_chunk(0);
+	}
+	__syncthreads();
+	for (; c < nChunks; c += gridDim.x) {
+		const uint64_t next = c + gridDim.x;
+		if (next < nChunks)
+			load_chunk(next);
+		const double* xs = lds + buf * BUF;
+		const double* wts = xs + CH * STRIDE;
+#pragma unroll 1
+		for (int kk = 0; kk < CH / 8; ++kk) {
+			const int row = (2 * kk + h) * 4 + lk;
+			const double wt = WEIGHTED ? wts[row] : 1.;
+			win_step<RECT, G, WEIGHTED>(xs + row * STRIDE + li, wt, acc, std::make_integer_sequence<int, NG>{});
+		}
+		if (next < nChunks)
+			store_chunk(buf ^ 1);
+		__syncthreads();
+		buf ^= 1;
+	}
+
+	// hand-over as in the grouped kernel (one round), then the pair's sums go to
+	// their tiles of the full triangle (padding tiles past the row are dropped)
+	double* red = lds + G * kMaxG * 256;
+	double* out = a.partials + (size_t)blockIdx.x * a.nTiles * 256;
+	if (h == 1) {
+#pragma unroll
+		for (int q = 0; q < NG; ++q)
+#pragma unroll
+			for (int v = 0; v < 4; ++v)
+				red[(q * 4 + v) * 64 + lane] = acc[q][v];
+	}
+	__syncthreads();
+	if (h == 0) {
+#pragma unroll
+		for (int q = 0; q < NG; ++q) {
+			const int t = win_begin(RECT, G) + q;
+			const uint32_t I = a.winI0 + win_row(RECT, t);
+			const uint32_t J = RECT ? a.winJ0 + (win_col(RECT, t) - kWin) : a.winI0 + win_col(RECT, t);
+			if (J >= a.nT)
+				continue;
+			double* o = out + (size_t)(I * a.nT - I * (I - 1) / 2 + (J - I)) * 256;
+#pragma unroll
+			for (int v = 0; v < 4; ++v) {
+				const int i = lk + 4 * v;
+				double val = acc[q][v] + red[(q * 4 + v) * 64 + lane];
+				if (a.accumulate)
+					val += o[i * 16 + li];
+				o[i * 16 + li] = val;
+			}
+		}
+	}
+	__syncthreads();
+	if (tid < CH)
+		wsh[tid] = wacc;
+	__syncthreads();
+	if (a.wsum && a.tileBase == 0 && tid == 0) {
+		double s = 0.;
+		for (int r = 0; r < CH; ++r)
+			s += wsh[r];
+		a.wsum[blockIdx.x] = a.accumulate ? a.wsum[blockIdx.x] + s : s;
+	}
+}
+
+template <bool RECT, bool WEIGHTED>
+__global__ __launch_bounds__(kIntThreads) void integral_win_kernel(IntArgs a)
+{
+	extern __shared__ double lds[];
+	__shared__ double wsh[64];
+	switch ((threadIdx.x >> 6) & 3) {
+	case 0: win_body<RECT, WEIGHTED, 0>(a, lds, wsh); break;
+	case 1: win_body<RECT, WEIGHTED, 1>(a, lds, wsh); break;
+	case 2: win_body<RECT, WEIGHTED, 2>(a, lds, wsh); break;
+	default: win_body<RECT, WEIGHTED, 3>(a, lds, wsh); break;
+	}
+}
+
 // fixed-order column sums of a [nRows][nCols] array
 constexpr int kColElems = 64;
 constexpr int kColChunks = 16;
@@ -466,26 +530,6 @@ __global__ __launch_bounds__(kColElems* kColChunks) void column_sum_kernel(
 }  // namespace
 
 typedef void (*IntKernel)(IntArgs);
-template <int CH, bool WT>
-static IntKernel pick_int_nt(uint32_t per)
-{
-	if (per <= 1) return integral_kernel<1, CH, WT>;
-	if (per <= 2) return integral_kernel<2, CH, WT>;
-	if (per <= 4) return integral_kernel<4, CH, WT>;
-	if (per <= 6) return integral_kernel<6, CH, WT>;
-	if (per <= 9) return integral_kernel<9, CH, WT>;
-	return integral_kernel<12, CH, WT>;
-}
-
-template <bool WT>
-static IntKernel pick_int_chunk(uint32_t per, int chunk)
-{
-	if (chunk >= 32) return pick_int_nt<32, WT>(per);
-	if (chunk >= 16) return pick_int_nt<16, WT>(per);
-	if (chunk >= 8) return pick_int_nt<8, WT>(per);
-	return pick_int_nt<4, WT>(per);
-}
-
 template <bool WT>
 static IntKernel pick_grp(uint32_t nT)
 {
@@ -508,11 +552,17 @@ static IntKernel pick_grp(uint32_t nT)
 static int grp_chunk_of(uint32_t nT) { return grp_chunk((int)nT); }
 
 struct IntegralPlan {
-	uint32_t D, Dp, nT, nTiles, ldsStride, tilesPerPass;
+	uint32_t D, Dp, nT, nTiles, ldsStride;
 	int chunk;
 	int nWg;
 	size_t lds;
-	IntKernel kern;  // grouped kernel (one pass) or generic kernel (passes of tilesPerPass tiles)
+	IntKernel kern = nullptr;  // grouped kernel (one pass over every tile)
+	// windowed passes (rows of more than 16 kGrpMaxBlocks doubles): diagonal and
+	// off-diagonal window kernels, their LDS row strides and dynamic LDS
+	bool windowed = false;
+	IntKernel kernDiag = nullptr, kernRect = nullptr;
+	uint32_t strideDiag = 0, strideRect = 0;
+	size_t ldsDiag = 0, ldsRect = 0;
 };
 
 // D = real row length (2 x complex values per row)
@@ -525,27 +575,66 @@ static int integral_plan(int device, uint32_t D, bool weighted, IntegralPlan& p)
 	p.ldsStride = p.Dp + 1;  // odd stride spreads the 4 rows of a read over the banks
 	p.kern = weighted ? pick_grp<true>(p.nT) : pick_grp<false>(p.nT);
 	if (p.kern) {
-		p.tilesPerPass = p.nTiles;
 		p.chunk = grp_chunk_of(p.nT);
 		p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
 	} else {
-		p.tilesPerPass = std::min<uint32_t>(p.nTiles, (uint32_t)(kIntWaves * kMaxTilesPerWave));
-		p.chunk = 32;
-		while (p.chunk > 4 && (size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread)
-			p.chunk /= 2;
-		if ((size_t)p.chunk * p.D / 2 > (size_t)kIntThreads * kMaxLoadsPerThread) {
-			set_error("integral: at most " + std::to_string(kIntThreads * kMaxLoadsPerThread / 8) +
-			          " complex values per row");
-			return RPWA_ERR_UNSUPPORTED;
-		}
-		p.lds = 2 * ((size_t)p.chunk * p.ldsStride + p.chunk) * sizeof(double);
-		const uint32_t per = (p.tilesPerPass + kIntWaves - 1) / kIntWaves;
-		p.kern = weighted ? pick_int_chunk<true>(per, p.chunk) : pick_int_chunk<false>(per, p.chunk);
+		// wider rows (the Hessian's derivative rows): windowed passes
+		p.windowed = true;
+		p.kernDiag = weighted ? integral_win_kernel<false, true> : integral_win_kernel<false, false>;
+		p.kernRect = weighted ? integral_win_kernel<true, true> : integral_win_kernel<true, false>;
+		p.strideDiag = 16 * win_blocks_of(false) + 1;
+		p.strideRect = 16 * win_blocks_of(true) + 1;
+		p.ldsDiag = (size_t)win_lds_doubles(false) * sizeof(double);
+		p.ldsRect = (size_t)win_lds_doubles(true) * sizeof(double);
+		p.chunk = win_chunk(true);
+		p.lds = std::max(p.ldsDiag, p.ldsRect);
 	}
 	int nCu = 0;
 	RPWA_HIP_CHECK(hipDeviceGetAttribute(&nCu, hipDeviceAttributeMultiprocessorCount, device));
 	p.nWg = nCu > 0 ? nCu : 1;
 	return RPWA_OK;
+}
+
+// Launch the rank-k update's passes on stream s: one grouped launch, or the
+// windowed diagonal and off-diagonal passes (every tile of the triangle written
+// by exactly one pass).
+static hipError_t launch_plan(const IntegralPlan& p, IntArgs a, hipStream_t s)
+{
+	auto allow = [](IntKernel k, size_t lds) -> hipError_t {
+		// dynamic LDS beyond 64 KiB is requested explicitly
+		if (lds + 64 * sizeof(double) > 64u * 1024u)
+			return hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+			                           (int)lds);
+		return hipSuccess;
+	};
+	if (p.windowed) {
+		if (hipError_t e = allow(p.kernDiag, p.ldsDiag))
+			return e;
+		if (hipError_t e = allow(p.kernRect, p.ldsRect))
+			return e;
+		const uint32_t nWin = (p.nT + kWin - 1) / kWin;
+		uint32_t pass = 0;
+		for (uint32_t u = 0; u < nWin; ++u)
+			for (uint32_t v = u; v < nWin; ++v, ++pass) {
+				const bool rect = v > u;
+				a.tileBase = pass;
+				a.winI0 = u * kWin;
+				a.winJ0 = v * kWin;
+				a.winCol0 = 16 * u * kWin;
+				a.winCol1 = 16 * v * kWin;
+				a.ldsStride = rect ? p.strideRect : p.strideDiag;
+				hipLaunchKernelGGL(rect ? p.kernRect : p.kernDiag, dim3(p.nWg), dim3(kIntThreads),
+				                   rect ? p.ldsRect : p.ldsDiag, s, a);
+				if (hipError_t e = hipGetLastError())
+					return e;
+			}
+		return hipSuccess;
+	}
+	if (hipError_t e = allow(p.kern, p.lds))
+		return e;
+	a.tileBase = 0;
+	hipLaunchKernelGGL(p.kern, dim3(p.nWg), dim3(kIntThreads), p.lds, s, a);
+	return hipGetLastError();
 }
 
 static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_t r, uint32_t c)
@@ -625,7 +714,7 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	auto fail = [&](int code) {
 		if (code != RPWA_OK)
 			(void)hipStreamSynchronize(s);  // nothing of this call may still run on the work area
-		hipFree(d_stage[0]); hipFree(d_stage[1]); hipFree(d_wstage[0]); hipFree(d_wstage[1]);
+		(void)hipFree(d_stage[0]); (void)hipFree(d_stage[1]); (void)hipFree(d_wstage[0]); (void)hipFree(d_wstage[1]);
 		return code;
 	};
 #define INT_CHECK(expr)                                                               \
@@ -642,23 +731,8 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	double* d_part = work.part;
 	double* d_wsum = work.wsum;
 	double* d_red = work.red;
-	IntKernel k = p.kern;
-	// dynamic LDS beyond 64 KiB (the two chunk buffers) is requested explicitly
-	if (p.lds + 64 * sizeof(double) > 64u * 1024u)
-		INT_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-		                              (int)p.lds));
 	// every pass re-reads the rows for its own set of output tiles
-	auto launch_passes = [&](IntArgs aa) -> hipError_t {
-		for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
-			aa.tileBase = tb;
-			aa.tileEnd = std::min(p.nTiles, tb + p.tilesPerPass);
-			hipLaunchKernelGGL(k, dim3(p.nWg), dim3(kIntThreads), p.lds, s, aa);
-			hipError_t e = hipGetLastError();
-			if (e != hipSuccess)
-				return e;
-		}
-		return hipSuccess;
-	};
+	auto launch_passes = [&](IntArgs aa) -> hipError_t { return launch_plan(p, aa, s); };
 	IntArgs a;
 	a.D = p.D; a.Dp = p.Dp; a.ldsStride = p.ldsStride; a.nT = p.nT; a.nTiles = p.nTiles;
 	a.partials = d_part; a.wsum = d_wsum; a.directWeights = directWeights ? 1 : 0;
@@ -692,8 +766,8 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 			first = false;
 		}
 		INT_CHECK(hipStreamSynchronize(s));
-		hipEventDestroy(done[0]);
-		hipEventDestroy(done[1]);
+		(void)hipEventDestroy(done[0]);
+		(void)hipEventDestroy(done[1]);
 	}
 	const int nb = (int)((p.nTiles * 256 + kColElems - 1) / kColElems);
 	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, s, d_part, p.nWg,
@@ -902,10 +976,6 @@ static int integral_noalloc(int device, const double* d_amps, uint64_t M, uint32
 	if (int st = integral_plan(device, 2 * W, weighted, p))
 		return st;
 	p.nWg = std::min(p.nWg, kNaWg);
-	IntKernel k = p.kern;
-	if (p.lds + 64 * sizeof(double) > 64u * 1024u)
-		RPWA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
 	IntArgs a;
 	a.x = d_amps;
 	a.w = d_w;
@@ -919,12 +989,7 @@ static int integral_noalloc(int device, const double* d_amps, uint64_t M, uint32
 	a.wsum = base + kNaWsum;
 	a.accumulate = 0;
 	a.directWeights = 0;
-	for (uint32_t tb = 0; tb < p.nTiles; tb += p.tilesPerPass) {
-		a.tileBase = tb;
-		a.tileEnd = std::min(p.nTiles, tb + p.tilesPerPass);
-		hipLaunchKernelGGL(k, dim3(p.nWg), dim3(kIntThreads), p.lds, 0, a);
-		RPWA_HIP_CHECK(hipGetLastError());
-	}
+	RPWA_HIP_CHECK(launch_plan(p, a, 0));
 	const int nb = (int)((p.nTiles * 256 + kColElems - 1) / kColElems);
 	hipLaunchKernelGGL(column_sum_kernel, dim3(nb), dim3(kColElems * kColChunks), 0, 0, base, p.nWg,
 	                   p.nTiles * 256, base + kNaRed);

@@ -209,10 +209,12 @@ def test_integral_matches_oracle(gpu):
 def test_integral_several_chunks_per_workgroup(gpu):
     """Every workgroup of the persistent grid walks at least three event chunks (up to 64 events each), so
     the staged double buffer, the accumulation over chunks and the ragged last chunk are all exercised:
-    5 column blocks (W=40), 11 (W=88), 12 (W=96: the pair hand-over in two rounds), with and without weights."""
+    5 column blocks (W=40), 11 (W=88), 12 (W=96: the pair hand-over in two rounds), and the windowed passes of
+    wider rows: 16 blocks (W=128, two windows of 8) and 19 (W=150, three windows, the last one padded), with and
+    without weights."""
     rng = np.random.default_rng(21)
     n_wg = gpu.device_info(0)["cus"]  # one workgroup per CU
-    for W in (40, 88, 96):
+    for W in (40, 88, 96, 128, 150):
         M = 3 * 64 * n_wg + 17
         print("integral W", W, "M", M, flush=True)
         a = rng.normal(size=(M, W)) + 1j * rng.normal(size=(M, W))
