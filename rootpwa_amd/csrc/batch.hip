@@ -3,7 +3,7 @@
 // One evaluation of all fits is four kernels, captured once in a hipGraph and
 // replayed for every L-BFGS iteration:
 //   batch_event_kernel   persistent waves; each wave takes work units (bin,
-//                        chunk of 64 tiles of 32 events, group of 16 starts)
+//                        chunk of 32 tiles of 32 events, group of 16 starts)
 //                        and computes, tile by tile, S = A T (FP32 MFMA),
 //                        sigma, ln sigma (FP64) and G += A^H (S / sigma) (FP32
 //                        MFMA), writing one partial per unit;
@@ -42,7 +42,7 @@ namespace {
 constexpr int kCols = 64;        // starts per bin: 4 column groups
 constexpr int kGroup = 16;       // starts per column group (one wave: 32 MFMA columns)
 constexpr int kEv = 32;          // events per tile (MFMA 32x32 rows)
-constexpr int kChunkTiles = 64;  // tiles per work unit (2048 events)
+constexpr int kChunkTiles = 32;  // tiles per work unit (1024 events): short units keep a batched fit's tail short
 constexpr int kThreads = 256;    // 4 independent waves per block
 constexpr int kMaxMem = 16;      // L-BFGS correction pairs
 constexpr int kTStride = kGroup + 1;  // LDS row of one wave's production amplitudes (float2)
