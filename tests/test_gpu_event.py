@@ -224,6 +224,27 @@ def test_integral_several_chunks_per_workgroup(gpu):
         assert H.rel_err(gpu.integral(a, w), O.integral(a, w)) <= 1e-12, W
 
 
+def test_integral_host_array_in_pieces(gpu):
+    """rpwa_integral streams a host array through the device in pieces of 256 MB: three pieces here (the last one
+    ragged), so the kernels add to the partial tiles of the earlier pieces and the weight sums accumulate; 88 waves
+    (grouped kernel) and 150 waves (windowed passes), with importance weights (ampIntegralMatrix.cc:477-483:
+    sums renormalised by the mean weight 1/w) and without. Reference: the product A^T conj(A) in numpy."""
+    rng = np.random.default_rng(33)
+    for W in (88, 150):
+        piece = (256 << 20) // (8 * 2 * W)  # events per streamed piece
+        M = 2 * piece + 12345
+        print("host integral W", W, "M", M, flush=True)
+        a = rng.normal(size=(M, W)) + 1j * rng.normal(size=(M, W))
+        a *= np.linspace(0.2, 1.8, W)[None, :]
+        ref = a.T @ a.conj()
+        assert H.rel_err(gpu.integral(a), ref) <= 1e-12, W
+        w = rng.uniform(0.5, 2.0, size=M)
+        wt = 1.0 / w
+        refw = (a * wt[:, None]).T @ a.conj() * (M / wt.sum())
+        assert H.rel_err(gpu.integral(a, w), refw) <= 1e-12, W
+        del a
+
+
 def test_integral_incoherent_sub_amplitudes(gpu):
     """ampIntegralMatrix with two incoherent sub-amplitudes per wave
     (ampIntegralMatrix.cc:432-466), host and device input."""
