@@ -122,6 +122,23 @@ def _d(a):
     return a.ctypes.data_as(_dp)
 
 
+_RAW = {}
+
+
+def _raw(name):
+    """The library function without declared argument types, for the per-
+    evaluation calls: the caller passes ctypes objects of the exact C types
+    (the handle as c_void_p, pointers as ctypes pointers), which skips the
+    per-argument converters (about 1 us of a C0 evaluation's 25 us)."""
+    f = _RAW.get(name)
+    if f is None:
+        _lib()
+        f = ctypes.CDLL(capi.LIB_PATH, handle=capi.lib()._handle)[name]
+        f.restype = _i
+        _RAW[name] = f
+    return f
+
+
 def _names(names):
     arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
     return arr
@@ -446,31 +463,39 @@ class pwaLikelihood:
     def _eval_buffers(self, par):
         """The parameters copied into a buffer of this object whose C pointer is
         kept (converting a numpy array to a ctypes pointer costs microseconds,
-        as much as an evaluation at small sizes), and the gradient buffer."""
+        as much as an evaluation at small sizes), the gradient buffer and the
+        value's c_double with its reference."""
         b = getattr(self, "_evalBufs", None)
         n = getattr(self, "_nPars", None)
         if b is None or n is None or b[0].size != n:
             n = n if n is not None else self.nmbPars()
-            pb, gb = np.zeros(n), np.zeros(n)
-            b = self._evalBufs = (pb, gb, _d(pb), _d(gb))
-        np.copyto(b[0], self._par(par))
+            pb, gb, v = np.zeros(n), np.zeros(n), ctypes.c_double()
+            b = self._evalBufs = (pb, gb, _d(pb), _d(gb), v, ctypes.byref(v))
+        if type(par) is np.ndarray and par.shape == b[0].shape:
+            np.copyto(b[0], par)
+        else:
+            np.copyto(b[0], self._par(par))
         return b
 
     def DoEval(self, par):
-        pb, _, pp, _ = self._eval_buffers(par)
-        v = ctypes.c_double()
-        _call("rpwa_pwalik_doeval", self.h, pp, ctypes.byref(v))
+        _, _, pp, _, v, vr = self._eval_buffers(par)
+        rc = _raw("rpwa_pwalik_doeval")(self.h, pp, vr)
+        if rc:
+            capi.check("rpwa_pwalik_doeval", rc)
         return v.value
 
     def Gradient(self, par):
-        pb, gb, pp, gp = self._eval_buffers(par)
-        _call("rpwa_pwalik_gradient", self.h, pp, gp)
+        _, gb, pp, gp, _, _ = self._eval_buffers(par)
+        rc = _raw("rpwa_pwalik_gradient")(self.h, pp, gp)
+        if rc:
+            capi.check("rpwa_pwalik_gradient", rc)
         return gb.copy()
 
     def FdF(self, par):
-        pb, gb, pp, gp = self._eval_buffers(par)
-        v = ctypes.c_double()
-        _call("rpwa_pwalik_fdf", self.h, pp, ctypes.byref(v), gp)
+        _, gb, pp, gp, v, vr = self._eval_buffers(par)
+        rc = _raw("rpwa_pwalik_fdf")(self.h, pp, vr, gp)
+        if rc:
+            capi.check("rpwa_pwalik_fdf", rc)
         return v.value, gb.copy()
 
     def timeFdF(self, par, n):
