@@ -63,7 +63,7 @@ struct BatchEventArgs {
 	const int32_t* colMap;        // [nBins][64] start held by each column (-1: none)
 	const float* T32;             // [nBins][64][rowLen] by start
 	const double* flat2;          // [nBins][64] by start
-	float* partG;                 // [nChunks * 4][Wp][16][2]
+	float* partG;                 // [nChunks * 4][16][Wp][2]: per unit, start slot-major
 	double* partLn;               // [nChunks * 4][16]
 	double* partInv;              // [nChunks * 4][16]
 };
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(kThreads, 2) void batch_event_kernel(const BatchEve
 				const float p0 = swap_pair(own0), p1 = swap_pair(own1);
 				if (cout == 0 && row0 < (int)rowLen) {
 					const int i = row0 >> 1;
-					pg[(i * kGroup + (j >> 1)) * 2 + 0] = own0 + p1;
-					pg[(i * kGroup + (j >> 1)) * 2 + 1] = p0 - own1;
+					pg[((j >> 1) * Wp + i) * 2 + 0] = own0 + p1;
+					pg[((j >> 1) * Wp + i) * 2 + 1] = p0 - own1;
 				}
 			}
 		double lnv = log(mant) + (double)expo * 0.69314718055994530942;
@@ -364,7 +364,9 @@ __global__ __launch_bounds__(kRedThreads) void batch_reduce_kernel(const BatchRe
 		const uint32_t nc = c1 - c0;
 		const uint32_t ca = c0 + (uint32_t)(((uint64_t)nc * h) / kRedParts);
 		const uint32_t cb = c0 + (uint32_t)(((uint64_t)nc * (h + 1)) / kRedParts);
-		const float* pg = r.partG + ((size_t)(ca * 4 + q) * r.Wp + i) * (2 * kGroup) + 2 * jc;
+		// fit-major partials: a fit's wave components are contiguous, so its reads
+		// use whole cache lines (with [wave][start] they took 8 bytes of each)
+		const float* pg = r.partG + (size_t)(ca * 4 + q) * r.Wp * (2 * kGroup) + ((size_t)jc * r.Wp + i) * 2;
 		uint32_t c = ca;
 		for (; c + 8 <= cb; c += 8, pg += 8 * cstride) {
 			float2 v[8];
