@@ -116,7 +116,11 @@ int rpwa_lik_collect(rpwa_lik_ctx* ctx, double* logLikelihood, double* derivativ
 /* Event part of the Hessian (pwaLikelihood.cc:697-753): for a, b over the
  * [rank][2][nmbWavesMax] production amplitudes, hessian[a][b][3] holds the
  * real/real, real/imag and imag/imag event sums; flatTerms[a] (complex) the
- * mixed flat-wave terms and hessianFlat the flat/flat term. */
+ * mixed flat-wave terms and hessianFlat the flat/flat term.  The per-event rows
+ * live in a device work area shared by the calls on one GPU (serialised),
+ * grown to the largest call and kept until the process ends: (nEvents or 8 GiB
+ * worth of rows) x 16 (rank W + W + 2) bytes, 8.5 GB at 2M events x 345
+ * parameters. */
 int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProdAmps,
                      double prodAmpFlat, uint32_t rank, double* hessian, double* flatTerms,
                      double* hessianFlat);

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace rpwa_amd {
@@ -101,6 +102,27 @@ __global__ __launch_bounds__(256) void hessian_rows_kernel(const void* amps, uin
 	}
 }
 
+// Per-device work area of the Hessian (the per-event rows, their weights and
+// the production amplitudes), kept between calls and grown on demand: a call
+// then allocates nothing on the device (allocating and freeing the 8.5 GB of
+// rows at 2M events x 345 parameters took about 2 ms of a 13 ms call).  Calls
+// on one device are serialised by the mutex; the area lives until the process
+// ends.
+struct HessWork {
+	std::mutex m;
+	char* buf = nullptr;
+	size_t cap = 0;
+};
+constexpr int kHessDevices = 64;
+
+HessWork& hess_work(int device)
+{
+	static HessWork work[kHessDevices];
+	return work[device];
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 }  // namespace
 }  // namespace rpwa_amd
 
@@ -136,19 +158,34 @@ extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint3
 		// event chunk: row buffers of at most ~8 GiB (2M events x 345 parameters in two chunks)
 		const uint64_t chunk = std::max<uint64_t>(1024, (8ull << 30) / (16ull * (nd + 1)));
 		const uint64_t cn = std::min<uint64_t>(chunk, lay.nEvents);
-		double2 *dRows = nullptr, *aRows = nullptr, *prod = nullptr;
-		double *wD = nullptr, *wA = nullptr;
-		auto cleanup = [&]() {
-			hipFree(dRows); hipFree(aRows); hipFree(prod); hipFree(wD); hipFree(wA);
-		};
-		if (hipMalloc(&dRows, cn * (nd + 1) * sizeof(double2)) != hipSuccess ||
-		    hipMalloc(&aRows, cn * (W + 1) * sizeof(double2)) != hipSuccess ||
-		    hipMalloc(&prod, (size_t)nmbProdAmps * sizeof(double2)) != hipSuccess ||
-		    hipMalloc(&wD, cn * sizeof(double)) != hipSuccess || hipMalloc(&wA, cn * sizeof(double)) != hipSuccess) {
-			cleanup();
-			set_error("rpwa_lik_hessian: device allocation failed");
-			return RPWA_ERR_NOMEM;
+		if (ctx->device < 0 || ctx->device >= kHessDevices) {
+			set_error("rpwa_lik_hessian: device index out of range");
+			return RPWA_ERR_INVALID;
 		}
+		// one allocation: rows d [cn][nd + 1], rows a [cn][W + 1] (complex), T, weights wD, wA [cn]
+		const size_t offA = align256(cn * (nd + 1) * sizeof(double2));
+		const size_t offP = offA + align256(cn * (W + 1) * sizeof(double2));
+		const size_t offWD = offP + align256((size_t)nmbProdAmps * sizeof(double2));
+		const size_t offWA = offWD + align256(cn * sizeof(double));
+		const size_t need = offWA + align256(cn * sizeof(double));
+		HessWork& work = hess_work(ctx->device);
+		std::lock_guard<std::mutex> lock(work.m);
+		if (need > work.cap) {
+			(void)hipFree(work.buf);
+			work.buf = nullptr;
+			work.cap = 0;
+			if (hipMalloc(&work.buf, need) != hipSuccess) {
+				work.buf = nullptr;
+				set_error("rpwa_lik_hessian: device allocation failed");
+				return RPWA_ERR_NOMEM;
+			}
+			work.cap = need;
+		}
+		double2* const dRows = reinterpret_cast<double2*>(work.buf);
+		double2* const aRows = reinterpret_cast<double2*>(work.buf + offA);
+		double2* const prod = reinterpret_cast<double2*>(work.buf + offP);
+		double* const wD = reinterpret_cast<double*>(work.buf + offWD);
+		double* const wA = reinterpret_cast<double*>(work.buf + offWA);
 		hipError_t e = hipMemcpy(prod, prodAmps, (size_t)nmbProdAmps * sizeof(double2), hipMemcpyHostToDevice);
 		// events per block of the row kernel: their rows and S within 48 KB of LDS
 		const size_t perEvent = ((size_t)W + 2 * rank) * sizeof(double2);
@@ -184,7 +221,6 @@ extern "C" int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint3
 						CA[i] += part[i];
 			}
 		}
-		cleanup();
 		RPWA_HIP_CHECK(e);
 		if (st != RPWA_OK)
 			return st;

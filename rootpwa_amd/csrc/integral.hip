@@ -637,18 +637,6 @@ static hipError_t launch_plan(const IntegralPlan& p, IntArgs a, hipStream_t s)
 	return hipGetLastError();
 }
 
-static double tileC(const std::vector<double>& C, const IntegralPlan& p, uint32_t r, uint32_t c)
-{
-	uint32_t I = r / 16, J = c / 16;
-	if (I > J) {
-		std::swap(r, c);
-		std::swap(I, J);
-	}
-	// row-by-row enumeration of the upper triangle
-	const uint32_t t = I * p.nT - I * (I - 1) / 2 + (J - I);
-	return C[(size_t)t * 256 + (r % 16) * 16 + (c % 16)];
-}
-
 // Per-device work area of the rank-k update (workgroup partial tiles, weight
 // sums, reduced tiles) and its stream, kept between calls and grown on demand:
 // a call then allocates nothing of its own on the device (an allocation and a
@@ -777,10 +765,22 @@ int syrk_run(int device, const double* d_x, uint64_t M, uint32_t D, const double
 	INT_CHECK(hipMemcpyAsync(wsums.data(), d_wsum, wsums.size() * sizeof(double), hipMemcpyDeviceToHost, s));
 	INT_CHECK(hipStreamSynchronize(s));
 #undef INT_CHECK
+	// the symmetric matrix from the upper-triangle tiles, tile by tile: a
+	// diagonal tile as computed (with weights its two halves can differ in the
+	// last bit), an off-diagonal tile at (I, J) and transposed at (J, I)
 	full.assign((size_t)D * D, 0.);
-	for (uint32_t r = 0; r < D; ++r)
-		for (uint32_t c = 0; c < D; ++c)
-			full[(size_t)r * D + c] = tileC(C, p, r, c);
+	for (uint32_t I = 0, t = 0; I < p.nT; ++I)
+		for (uint32_t J = I; J < p.nT; ++J, ++t) {
+			const double* tile = C.data() + (size_t)t * 256;
+			const uint32_t r1 = std::min<uint32_t>(16, D - 16 * I), c1 = std::min<uint32_t>(16, D - 16 * J);
+			for (uint32_t r = 0; r < r1; ++r)
+				for (uint32_t c = 0; c < c1; ++c) {
+					const uint32_t row = 16 * I + r, col = 16 * J + c;
+					full[(size_t)row * D + col] = tile[r * 16 + c];
+					if (I != J)
+						full[(size_t)col * D + row] = tile[r * 16 + c];
+				}
+		}
 	if (wtot) {
 		double t = 0.;
 		for (double v : wsums)
