@@ -736,10 +736,11 @@ __global__ __launch_bounds__(kThreads, 2) void stage_kernel(StageArgs a)
 }
 
 // ---- The wave-tile variant (complex64, rank <= 2, rows of at most 16 NG units,
-// tiles of 64 events): each wave takes 16 events of the staged tile and computes
-// their complete S on v_mfma_f64_4x4x4f64, sigma and u inside the wave, and the
-// gradient of those events on v_mfma_f32_4x4x1f32 -- two barriers per tile (the
-// stage) and no single-wave step.
+// tiles of 64 events): each wave stages its own 16 events of the tile and
+// computes their complete S on v_mfma_f64_4x4x4f64, sigma and u inside the wave,
+// and the gradient of those events on v_mfma_f32_4x4x1f32 -- no workgroup
+// barrier in the tile loop (with the tile staged by the whole workgroup, the two
+// barriers per tile cost 3.5% of the kernel time) and no single-wave step.
 //
 // S: 4 blocks of 4x4x4 per MFMA, K = the four floats of one 16-byte unit (Re, Im
 // of its two waves), block b and row i = event 4b + i of the wave's 16, column
@@ -813,6 +814,46 @@ __device__ __forceinline__ void g_mfma_block(const char* rows, uint32_t rowBytes
 			GD[g][0][i] += (double)acc[g][i];
 }
 
+// Per-wave staging of the wave-tile kernel: each wave stages only its own 16
+// rows of a tile (16 rowUnits units, KL loads of 16 bytes per lane) into its
+// rows of the stage and contracts them; no other wave reads them, so the tile
+// loop needs no workgroup barrier (a wave's LDS operations complete in order).
+template <int KL>
+__device__ __forceinline__ void wave_load_plan(const StageArgs& a, int lane, int wid, uint32_t (&ldsPos)[KL])
+{
+	const uint32_t waveUnits = 16u * a.rowUnits;
+	const uint32_t dumpUnit = 64u * a.rowUnitsP;
+#pragma unroll
+	for (int k = 0; k < KL; ++k) {
+		const uint32_t g = (uint32_t)lane + 64u * k;
+		const uint32_t row = g / a.rowUnits;
+		ldsPos[k] = g < waveUnits ? (16u * (uint32_t)wid + row) * a.rowUnitsP + (g - row * a.rowUnits) : dumpUnit;
+	}
+}
+
+template <int KL>
+__device__ __forceinline__ void fetch_wave_lean(const StageArgs& a, uint32_t t, uint32_t lastTile, int lane, int wid,
+                                                f32x4 (&buf)[KL])
+{
+	const uint32_t waveUnits = 16u * a.rowUnits;
+	const uint64_t base = (uint64_t)t * a.tileUnits + (uint64_t)wid * waveUnits;
+	if (t < lastTile) {
+		const f32x4* tb = a.amps + base;
+#pragma unroll
+		for (int k = 0; k < KL; ++k) {
+			const uint32_t g = (uint32_t)lane + 64u * k;
+			buf[k] = __builtin_nontemporal_load(tb + (g < waveUnits ? g : 0u));
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < KL; ++k) {
+			const uint32_t g = (uint32_t)lane + 64u * k;
+			const uint64_t gg = base + (g < waveUnits ? g : 0u);
+			buf[k] = __builtin_nontemporal_load(a.amps + (gg < a.lastUnit ? gg : a.lastUnit));
+		}
+	}
+}
+
 template <int R, int NG, int KL>
 __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 {
@@ -833,8 +874,8 @@ __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 	// KL loads of 16 bytes per thread and tile (64 rows of at most 4 KL units)
 	uint32_t ldsPacked[(KL + 1) / 2];
 	{
-		uint32_t ldsPos[KL], ldOff[KL];
-		load_plan<kThreads, KL, E>(a, tid, ldsPos, ldOff);
+		uint32_t ldsPos[KL];
+		wave_load_plan<KL>(a, lane, wid, ldsPos);
 		pack_positions<KL>(ldsPos, ldsPacked);
 	}
 	{
@@ -925,24 +966,29 @@ __global__ __launch_bounds__(kThreads, 2) void wave_tile_kernel(StageArgs a)
 			g_mfma_block<NG>(sb + (uint32_t)(wid * EW) * rowBytes, rowBytes, uw, aOff, bOff, GD);
 	};
 	if (nMine > 0)
-		fetch_tile_lean<kThreads>(a, tile0, lastTile, tid, bufA);
+		fetch_wave_lean<KL>(a, tile0, lastTile, lane, wid, bufA);
 	if (nMine > 1)
-		fetch_tile_lean<kThreads>(a, tile0 + step, lastTile, tid, bufB);
+		fetch_wave_lean<KL>(a, tile0 + step, lastTile, lane, wid, bufB);
+	lds_barrier();  // T' of every unit is in LDS
 	for (uint32_t it = 0; it < nMine; it += 2) {
-		lds_barrier();  // every wave is done with the previous tile
 		store_tile_packed<KL>(stage, ldsPacked, bufA);
 		if (it + 2 < nMine)
-			fetch_tile_lean<kThreads>(a, tile0 + (it + 2) * step, lastTile, tid, bufA);
-		lds_barrier();
+			fetch_wave_lean<KL>(a, tile0 + (it + 2) * step, lastTile, lane, wid, bufA);
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		contract(tile0 + it * step);
 		if (it + 1 < nMine) {
-			lds_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			store_tile_packed<KL>(stage, ldsPacked, bufB);
 			if (it + 3 < nMine)
-				fetch_tile_lean<kThreads>(a, tile0 + (it + 3) * step, lastTile, tid, bufB);
-			lds_barrier();
+				fetch_wave_lean<KL>(a, tile0 + (it + 3) * step, lastTile, lane, wid, bufB);
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			contract(tile0 + (it + 1) * step);
 		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
 	write_partial_row_gm<R, E, NG, kThreads, true>(a, smem, tid, lane, wid, GD, mant, expo, wsum, lsum);
 }
