@@ -28,6 +28,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3
 SEED = 20261202 + 1
+# kernel timing in the timed FdF loop: HIP events on the launches of every 8th evaluation
+TIMING_PERIOD = 8
 SEED_BATCH = 20261202 + 3
 STREAM_TRUE_T, STREAM_MC, STREAM_DATA, STREAM_START = 2, 3, 4, 5
 
@@ -153,9 +155,9 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    # the GPU clock settles under sustained load only after a few hundred
-    # evaluations (0.1 s: the event kernel runs ~3% slower before that)
-    ap.add_argument("--warmup", type=int, default=300)
+    # the event kernel settles under sustained load only after several hundred
+    # evaluations (0.1-0.2 s; it runs 3-5% slower before that)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--events", type=int, default=2_000_000)
     ap.add_argument("--mc-events", type=int, default=5_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="events of the single-thread CPU baseline")
@@ -262,8 +264,10 @@ def main():
     # ---- timed FdF evaluations ------------------------------------------------
     def timed(L, label):
         ctx = L.gpuContext()
-        # HIP events around every evaluation in the timed region: the kernel durations of the roofline
-        ctx.set_timing(True)
+        # HIP events bound to the launches of every TIMING_PERIOD-th evaluation in the timed
+        # region: the kernel durations of the roofline (events on every launch add ~15 us of
+        # host time to each evaluation; the bench reports that rate as well)
+        ctx.set_timing(TIMING_PERIOD)
         for _ in range(args.warmup):
             L.FdF(par)
         capi.synchronize()
@@ -298,8 +302,18 @@ def main():
             L.FdF(par)
         rate_default = args.steps / (time.perf_counter() - t)
         log("%s: %.1f evals/s without kernel events" % (label, rate_default))
+        # and with the events on the launches of every evaluation
+        ctx.set_timing(1)
+        capi.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            L.FdF(par)
+        rate_every = args.steps / (time.perf_counter() - t)
+        ctx.set_timing(False)
+        log("%s: %.1f evals/s with kernel events on every evaluation" % (label, rate_every))
         return dict(rate=rate, ms=1e3 * el / args.steps, kernel_s=k_avg, reduce_s=red / max(n, 1), f=f, g=g,
-                    info=info, pair_rate=n_pair / el_pair, rate_without_events=rate_default)
+                    info=info, pair_rate=n_pair / el_pair, rate_without_events=rate_default, timed_launches=n,
+                    rate_every_timed=rate_every)
 
     r_mix = timed(L_mix, "mixed (complex64)")
     r_64 = timed(L_64, "fp64 (complex128)")
@@ -415,10 +429,17 @@ def main():
                  "achieved_GBs": ach_64, "frac": ach_64 / HBM_PEAK_GBS, "frac_of_measured_read": ach_64 / read_gbs,
                  "traffic": traffic.get("fp64")},
         "mixed_over_fp64": r_mix["rate"] / r_64["rate"],
-        # the same FdF loop with kernel timing off (the library's default: no events,
-        # completion read from the reduction's flag); value keeps the timed loop above
+        # kernel timing of the timed loop (value, kernel_ms): HIP events bound to the
+        # launches of every TIMING_PERIOD-th evaluation; the same FdF loop with timing off
+        # (the library's default: no events, completion read from the reduction's flag)
+        # and with events on every evaluation
+        "kernel_timing": {"every_nth_evaluation": TIMING_PERIOD,
+                          "timed_launches": {"mixed": r_mix["timed_launches"], "fp64": r_64["timed_launches"]}},
         "without_kernel_events": {"mixed_evals_per_s": r_mix["rate_without_events"],
                                   "fp64_evals_per_s": r_64["rate_without_events"]},
+        "with_kernel_events_every_evaluation": {"mixed_evals_per_s": r_mix["rate_every_timed"],
+                                                "fp64_evals_per_s": r_64["rate_every_timed"],
+                                                "mixed_over_fp64": r_mix["rate_every_timed"] / r_64["rate_every_timed"]},
         "integral": {"ms": 1e3 * t_int, "tflops_algorithmic": int_flops / t_int / 1e12,
                      "frac_fp64_mfma": int_flops / t_int / 1e12 / FP64_MFMA_PEAK_TFS, "mc_events": M},
         "bandwidth_probe": {"read_GBs": read_gbs, "copy_GBs": copy_gbs, "bytes": 4 << 30},

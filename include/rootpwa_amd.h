@@ -127,15 +127,17 @@ int rpwa_lik_hessian(rpwa_lik_ctx* ctx, const double* prodAmps, uint32_t nmbProd
 
 /* Accumulated time spent in this context's kernels [s]
  * (likelihoodInterface::kernelTime / resetKernelTime, likelihoodInterface.cuh:124-125),
- * measured with HIP events bound to the kernel launches of every evaluation
- * while timing is on (the sum of the event kernels' and the reduction's own
- * device time, without launch gaps; rpwa_lik_set_timing, or
- * RPWA_KERNEL_TIMING=1 in the environment; off by default: the events cost host
- * time on every evaluation).  Either way an evaluation's completion is read
+ * measured with HIP events bound to the kernel launches of the timed
+ * evaluations while timing is on (the sum of the event kernels' and the
+ * reduction's own device time, without launch gaps; rpwa_lik_set_timing, or
+ * RPWA_KERNEL_TIMING=N in the environment; off by default: the events cost
+ * about 15 us of host time per timed evaluation).  rpwa_lik_kernel_times
+ * counts the timed evaluations.  Either way an evaluation's completion is read
  * from flags the reduction writes into host memory. */
 int rpwa_lik_kernel_time(rpwa_lik_ctx* ctx, double* seconds);
 int rpwa_lik_reset_kernel_time(rpwa_lik_ctx* ctx);
-/* Kernel timing on (enable != 0) or off for the following evaluations. */
+/* Kernel timing for the following evaluations: enable = N > 0 times every N-th
+ * evaluation of the context (1: every one), 0 (or negative) turns it off. */
 int rpwa_lik_set_timing(rpwa_lik_ctx* ctx, int enable);
 
 /* Split of the accumulated kernel time: event kernel and fixed-order

@@ -235,7 +235,10 @@ class LikelihoodContext:
         return a.value, b.value, n.value
 
     def set_timing(self, enable=True):
-        call("rpwa_lik_set_timing", self.h, int(bool(enable)))
+        """Kernel timing with HIP events: True or 1 times every evaluation, N > 1
+        every N-th, False or 0 turns it off."""
+        n = int(enable) if not isinstance(enable, bool) else int(enable)
+        call("rpwa_lik_set_timing", self.h, max(0, n))
 
     def reset_kernel_time(self):
         call("rpwa_lik_reset_kernel_time", self.h)

@@ -451,7 +451,8 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	a.sigma = c->d_sigma;
 	// timing events are bound to the launches: the first event launch starts ev0,
 	// the last one stops evMid, the reduction runs between evRed and ev1
-	hipEvent_t kStart = c->timing ? c->ev0 : nullptr, kStop = c->timing ? c->evMid : nullptr;
+	const bool timed = c->timing != 0 && (c->seq + 1) % c->timing == 0;
+	hipEvent_t kStart = timed ? c->ev0 : nullptr, kStop = timed ? c->evMid : nullptr;
 	if (!blocks) {
 		RPWA_HIP_CHECK(launch_event_kernel(c->lay, c->tile[slot], a, true, c->nWg[slot], c->stream, kStart, kStop));
 	} else {
@@ -469,9 +470,10 @@ int submit(rpwa_lik_ctx* c, const double* prodAmps, uint32_t nmbProdAmps, double
 	}
 	const unsigned long long seq = ++c->seq;
 	RPWA_HIP_CHECK(launch_event_reduce(c->d_partials, c->nWg[slot], nElem, flat, c->d_out, c->stream,
-	                                   c->d_flag, seq, c->timing ? c->evRed : nullptr,
-	                                   c->timing ? c->ev1 : nullptr));
+	                                   c->d_flag, seq, timed ? c->evRed : nullptr,
+	                                   timed ? c->ev1 : nullptr));
 	c->pendingBlocks = reduce_blocks(nElem);
+	c->pendingTimed = timed;
 	c->pending = true;
 	c->evaluated[slot] = true;
 	c->pendingRank = rank;
@@ -514,10 +516,10 @@ int collect(rpwa_lik_ctx* c, double* lnl, double* deriv, double* derivFlat, cons
 		RPWA_HIP_CHECK(hipStreamSynchronize(c->stream));
 	// the timing events complete with the reduction's dispatch, just after its
 	// flags: waiting for the last one costs little once the flags are seen
-	if (c->timing)
+	if (c->pendingTimed)
 		RPWA_HIP_CHECK(hipEventSynchronize(c->ev1));
 	float ms = 0.f, ms1 = 0.f;
-	if (c->timing && hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
+	if (c->pendingTimed && hipEventElapsedTime(&ms, c->ev0, c->evMid) == hipSuccess &&
 	    hipEventElapsedTime(&ms1, c->evRed, c->ev1) == hipSuccess) {
 		c->eventTime += 1e-3 * ms;
 		c->reduceTime += 1e-3 * ms1;
@@ -605,7 +607,7 @@ extern "C" int rpwa_lik_create(rpwa_lik_ctx** ctx, int device, int precision)
 	if (const char* env = std::getenv("RPWA_AUTOTUNE"))
 		c->autotune = std::string(env) != "0";
 	if (const char* env = std::getenv("RPWA_KERNEL_TIMING"))
-		c->timing = std::string(env) != "0";
+		c->timing = (uint32_t)std::max(0, std::atoi(env));
 	if (e != hipSuccess) {
 		set_error(std::string("rpwa_lik_create: ") + hipGetErrorString(e));
 		for (hipEvent_t ev : {c->ev0, c->ev1, c->evMid, c->evRed})
@@ -971,7 +973,7 @@ extern "C" int rpwa_lik_set_timing(rpwa_lik_ctx* ctx, int enable)
 		set_error("rpwa_lik_set_timing: an evaluation is pending");
 		return RPWA_ERR_STATE;
 	}
-	ctx->timing = enable != 0;
+	ctx->timing = enable > 0 ? (uint32_t)enable : 0u;
 	return RPWA_OK;
 }
 

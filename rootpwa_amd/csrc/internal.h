@@ -166,7 +166,8 @@ struct rpwa_lik_ctx {
 	unsigned long long* d_flag = nullptr;  // ... and their device view
 	uint32_t pendingBlocks = 0;            // blocks of the pending evaluation's reduction
 	unsigned long long seq = 0;            // evaluations submitted
-	bool timing = false;                   // per-evaluation kernel timing with events (rpwa_lik_set_timing)
+	uint32_t timing = 0;                   // kernel timing with events: every timing-th evaluation, 0 = off (rpwa_lik_set_timing)
+	bool pendingTimed = false;             // the pending evaluation's launches carry the timing events
 	size_t partialCap = 0;
 	// pending asynchronous evaluation (rpwa_lik_submit / rpwa_lik_collect)
 	bool pending = false;

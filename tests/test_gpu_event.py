@@ -318,4 +318,13 @@ def test_kernel_timing_leaves_results_unchanged(gpu, rank, prec):
     f, d, df = ctx.fdf(prod, 0.1)
     assert f == ref[0] and np.array_equal(d, ref[1])
     assert ctx.kernel_times()[2] == 4
+    # timing every third evaluation: 9 evaluations, 3 of them timed, results unchanged
+    ctx.reset_kernel_time()
+    ctx.set_timing(3)
+    for _ in range(9):
+        f, d, df = ctx.fdf(prod, 0.1)
+        assert f == ref[0] and np.array_equal(d, ref[1]) and df == ref[2]
+    ev3, red3, n3 = ctx.kernel_times()
+    assert n3 == 3 and ev3 > 0 and red3 > 0
+    ctx.set_timing(False)
     ctx.close()
