@@ -237,8 +237,7 @@ class LikelihoodContext:
     def set_timing(self, enable=True):
         """Kernel timing with HIP events: True or 1 times every evaluation, N > 1
         every N-th, False or 0 turns it off."""
-        n = int(enable) if not isinstance(enable, bool) else int(enable)
-        call("rpwa_lik_set_timing", self.h, max(0, n))
+        call("rpwa_lik_set_timing", self.h, max(0, int(enable)))
 
     def reset_kernel_time(self):
         call("rpwa_lik_reset_kernel_time", self.h)
